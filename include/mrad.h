@@ -1,0 +1,192 @@
+/*
+ * mrad: MI355X (gfx950) path-tracing hot path behind a plain C ABI.
+ *
+ * This is the drop-in boundary for the reference's CUDA render path.  Every
+ * entry point names the reference interface it replaces (paths relative to
+ * the reference repository, src/xyz/marsavic/gfxlab/).  Arguments are plain
+ * pointers and sizes; all arrays are borrowed and copied during the call.
+ *
+ * Status codes: MRAD_OK (0) or a negative MRAD_ERR_*; the message of the
+ * last failure on the calling thread is returned by mrad_last_error().  A
+ * Java binding maps MRAD_ERR_INVALID_ARGUMENT to IllegalArgumentException and
+ * MRAD_ERR_ILLEGAL_STATE / MRAD_ERR_UNAVAILABLE / MRAD_ERR_HIP to
+ * IllegalStateException, as GpuRayTracer does (gpu/GpuRayTracer.java:324-337,
+ * :470-476, :572-580).
+ *
+ * Threading: every call on one tracer handle is serialized by an internal
+ * mutex (the reference's synchronized(this), GpuRayTracer.java:339); each
+ * handle owns one HIP stream on its device.
+ */
+#ifndef MRAD_H
+#define MRAD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	MRAD_OK = 0,
+	MRAD_ERR_INVALID_ARGUMENT = -1,
+	MRAD_ERR_ILLEGAL_STATE = -2,
+	MRAD_ERR_UNAVAILABLE = -3,
+	MRAD_ERR_HIP = -4,
+	MRAD_ERR_OUT_OF_MEMORY = -5,
+	MRAD_ERR_UNSUPPORTED = -6
+};
+
+/* BVH build modes built here (gpu/BvhBuildMode.java:4-12). */
+enum { MRAD_BVH_UNIFORM_SAH = 0, MRAD_BVH_WEIGHTED_SAH = 1 };
+
+/* Primitive kinds (refs' 2-bit type, gpu/GpuKernelSources.java:19-24). */
+enum { MRAD_PRIM_SPHERE = 0, MRAD_PRIM_BOX = 1, MRAD_PRIM_AFFINE_SPHERE = 2, MRAD_PRIM_AFFINE_BOX = 3 };
+
+/* Counter slots of mrad_frame_stats.counters (GpuKernelSources.java:30-43). */
+enum {
+	MRAD_STAT_RAYS = 0, MRAD_STAT_AABB_TESTS, MRAD_STAT_PRIMITIVE_TESTS, MRAD_STAT_SPHERE_TESTS,
+	MRAD_STAT_BOX_TESTS, MRAD_STAT_PLANE_TESTS, MRAD_STAT_AFFINE_SPHERE_TESTS,
+	MRAD_STAT_AFFINE_BOX_TESTS, MRAD_STAT_STACK_OVERFLOWS, MRAD_STAT_MAX_STACK_SIZE,
+	MRAD_STAT_INTERNAL_NODES, MRAD_STAT_LEAF_NODES, MRAD_STAT_HOMOGENEOUS_LEAF_NODES,
+	MRAD_STAT_MIXED_LEAF_NODES, MRAD_STAT_COUNT
+};
+
+/* ------------------------------------------------------------------ scene */
+
+/* BvhBuildConfig record (gpu/BvhBuildConfig.java:6-15). */
+typedef struct {
+	int mode;                 /* MRAD_BVH_* */
+	int leafSize;
+	double traversalWeight;   /* PrimitiveCostModel.INTERIOR_TRAVERSAL = 1.275576713 */
+	double sphereWeight, boxWeight, affineSphereWeight, affineBoxWeight;
+	double lambda;            /* 0 for UNIFORM_SAH */
+} mrad_bvh_config;
+
+/* GpuScene.BvhStats (gpu/GpuScene.java:113-133), the fields the flat build has. */
+typedef struct {
+	int node_count, root_count, leaf_count, max_depth, primitive_ref_count, leaf_size;
+	double generalized_sah_cost, uniform_sah_cost, weighted_sah_cost;
+	int64_t bytes;
+	int min_leaf_occupancy, max_leaf_occupancy;
+	double mean_leaf_occupancy;
+	int64_t build_nanos;
+} mrad_bvh_stats;
+
+/* The reference's packed wire format (GpuScene.java:24-31, :507-607). */
+typedef struct {
+	const float *spheres; int sphere_count;                /* f32 x 5  */
+	const float *boxes; int box_count;                     /* f32 x 7  */
+	const float *planes; int plane_count;                  /* f32 x 26 */
+	const float *affine_spheres; int affine_sphere_count;  /* f32 x 22 */
+	const float *affine_boxes; int affine_box_count;       /* f32 x 22 */
+	const float *node_bounds; int node_count;              /* f32 x 6  */
+	const int32_t *node_data;                              /* i32 x 4  */
+	const int32_t *primitive_refs; int primitive_ref_count;
+	const int32_t *root_indices; int root_count;           /* <= 4, roots[0] == 0 */
+	const float *materials; int material_count;            /* f32 x 13 */
+	float background[3];
+	uint64_t revision;
+} mrad_scene_arrays;
+
+typedef struct mrad_scene mrad_scene;
+
+/* new GpuScene(config, options) (GpuScene.java:165-170). */
+int mrad_scene_create(const mrad_bvh_config *config, mrad_scene **out);
+void mrad_scene_destroy(mrad_scene *scene);
+/* GpuScene.setBackground (GpuScene.java:200-203). */
+int mrad_scene_set_background(mrad_scene *scene, double r, double g, double b);
+/* Materials are 13 doubles per primitive in packed order: diffuse.rgb,
+ * reflective.rgb, refractive.rgb, emittance.rgb, refractive index; equal
+ * materials are deduplicated (GpuScene.java:850-859). */
+/* GpuScene.addSphere (GpuScene.java:205-221): center.xyz, radius per sphere. */
+int mrad_scene_add_spheres(mrad_scene *scene, int n, const double *center_radius, const double *materials);
+/* GpuScene.addBox (GpuScene.java:223-233): min.xyz, max.xyz per box. */
+int mrad_scene_add_boxes(mrad_scene *scene, int n, const double *min_max, const double *materials);
+/* GpuScene.addPlane (GpuScene.java:235-253): point.xyz, normal.xyz per plane. */
+int mrad_scene_add_planes(mrad_scene *scene, int n, const double *point_normal, const double *materials);
+/* GpuScene.addAffineSphere / addAffineBox (GpuScene.java:457-479): 3x4
+ * inverse and inverse-transpose, world center, bounds min.xyz max.xyz. */
+int mrad_scene_add_affine(mrad_scene *scene, int kind, int n, const double *inverse12,
+                          const double *inverse_transpose12, const double *center3,
+                          const double *bounds6, const double *materials);
+/* GpuSceneBuilder.convertSolid for affine SceneSphere/SceneBox
+ * (gpu/GpuSceneBuilder.java:37-78, :130-159): forward 3x4 transform. */
+int mrad_scene_add_affine_transforms(mrad_scene *scene, int kind, int n, const double *transform12,
+                                     const double *materials);
+/* GpuScene.bvhData() (GpuScene.java:609-700): packs and builds lazily. */
+int mrad_scene_build(mrad_scene *scene);
+/* GpuScene.*Data(), bvh*() (GpuScene.java:507-607): views valid until the
+ * scene is modified or destroyed. */
+int mrad_scene_view(mrad_scene *scene, mrad_scene_arrays *out);
+/* GpuScene.bvhStats() (GpuScene.java:607). */
+int mrad_scene_bvh_stats(mrad_scene *scene, mrad_bvh_stats *out);
+
+/* ----------------------------------------------------------------- tracer */
+
+/* GpuRayTracer constructor arguments (gpu/GpuRayTracer.java:256-282) plus
+ * the -Dgfxlab.gpu.* properties it reads (:1100-1103, GpuLaunchProvenance). */
+typedef struct {
+	int width, height;
+	int samples_per_frame;
+	int bvh_traversal;        /* boolean: BVH (1) or linear scan (0) */
+	int collect_metrics;      /* boolean: counter variant of the kernels */
+	int bvh_stack_size;       /* gfxlab.gpu.bvhStackSize, 8..256 (0 = 32) */
+	int samples_per_launch;   /* physical launches per frame = ceil(spp / this); 0 = all */
+	int device;               /* HIP device ordinal */
+} mrad_tracer_config;
+
+/* GpuRayTracer.FrameStats (GpuRayTracer.java:79-99). */
+typedef struct {
+	int64_t total_nanos, kernel_nanos, maximum_physical_kernel_nanos, copy_nanos, upload_nanos;
+	uint64_t counters[MRAD_STAT_COUNT];
+	int physical_launches;
+} mrad_frame_stats;
+
+typedef struct mrad_tracer mrad_tracer;
+
+/* new GpuRayTracer(w, h, spp, bvh, metrics) (GpuRayTracer.java:256-282). */
+int mrad_create(const mrad_tracer_config *config, mrad_tracer **out);
+/* GpuRayTracer.close() (GpuRayTracer.java:1223-1310); NULL is a no-op. */
+void mrad_destroy(mrad_tracer *tracer);
+/* GpuRayTracer.isAvailable() (GpuRayTracer.java:284-287): 1 or 0. */
+int mrad_is_available(mrad_tracer *tracer);
+/* ensureSceneResident/uploadScene (GpuRayTracer.java:873-948): uploads and
+ * re-lays-out the wire arrays unless (scene_key, arrays->revision) is the
+ * resident pair. */
+int mrad_upload_scene(mrad_tracer *tracer, const mrad_scene_arrays *scene, uint64_t scene_key);
+/* Russian roulette after bounce start_depth with survival clamp(max T,
+ * 0.05, 0.95); -1 (the default) renders exactly the reference integrator.
+ * No counterpart in the reference: its integrator has no roulette. */
+int mrad_set_russian_roulette(mrad_tracer *tracer, int start_depth);
+/* GpuRayTracer.renderSample(rgb, scene, camera, maxDepth, frameIndex,
+ * frameSeed) (GpuRayTracer.java:317-432).  camera = position, forward,
+ * right, up (GpuCamera.java:8-22).  rgb_out (W*H*3 floats) may be NULL to
+ * leave the frame on the device (read it later with mrad_copy_frame). */
+int mrad_render_sample(mrad_tracer *tracer, float *rgb_out, const float camera[12], int max_depth,
+                       int frame_index, uint64_t frame_seed);
+/* The device-to-host copy of renderSample (GpuRayTracer.java:397-399). */
+int mrad_copy_frame(mrad_tracer *tracer, float *rgb_out);
+/* GpuRayTracer.traceReplay(scene, rays, n, metrics) (GpuRayTracer.java:593-716)
+ * for the resident scene; outputs as TraceReplayResult (:119-126).
+ * per_ray_counters (n x 14 u32, slot order as MRAD_STAT_*) may be NULL;
+ * it requires a tracer created with collect_metrics. */
+int mrad_trace_replay(mrad_tracer *tracer, const float *rays, int n, int collect_metrics,
+                      int32_t *hit_flags, float *hit_distances, int32_t *hit_primitive_orders,
+                      int32_t *hit_material_indices, float *hit_normals, uint32_t *per_ray_counters);
+/* GpuRayTracer.lastFrameStats() (GpuRayTracer.java:298-307). */
+int mrad_get_frame_stats(mrad_tracer *tracer, mrad_frame_stats *out);
+/* GpuRayTracer.deviceInfo() (GpuRayTracer.java:309-311): name and gfx arch. */
+int mrad_device_info(mrad_tracer *tracer, char *name, int name_len, char *arch, int arch_len);
+/* GpuRayTracer.compiledPtxSha256() (GpuRayTracer.java:314-316): SHA-256 hex
+ * (64 chars + NUL) of the gfx950 code object embedded in this library. */
+int mrad_code_object_sha256(char *out65);
+/* GpuRayTracer.validateFullPathDepth (GpuRayTracer.java:470-476). */
+int mrad_validate_full_path_depth(int max_depth);
+/* Message of the last failed call on this thread ("" after success). */
+const char *mrad_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRAD_H */

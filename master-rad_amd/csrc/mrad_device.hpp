@@ -1,0 +1,689 @@
+// Device-side arithmetic of the path-tracing hot path for gfx950.
+//
+// Semantics follow src/xyz/marsavic/gfxlab/gpu/GpuKernelSources.java; every
+// expression keeps the reference's operation order, and the library is built
+// with -ffp-contract=off so that only the fmaf calls the reference writes are
+// fused.  What differs from the reference is data layout and work placement:
+//
+//   * BVH nodes are 64-byte records: a parent holds both children's bounds
+//     (48 B) and its own {left|kind, right, first, count}, so one interior
+//     expansion is one aligned 64-byte fetch instead of 16 + 2 x 24 bytes in
+//     three places.
+//   * During traversal a primitive test computes only its candidate distance
+//     and the reference's "found" predicate.  Normal and material of the
+//     closest hit are recomputed once, after traversal, by the same
+//     arithmetic on the winning primitive, so the per-test fetch of an affine
+//     primitive is 48 bytes (inverse rows with the centre) instead of 88.
+//
+// Accept rules are the reference's: BVH leaves test against
+// nextafterf(best.t, +inf) and accept on (t, order) (GpuKernelSources.java:772-806);
+// the linear scan and planes test against best.t itself (:338-734, :1015-1018).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mrad {
+
+constexpr float kEps = 1e-4f;
+constexpr uint32_t kRefTypeMask = 0xC0000000u;
+constexpr uint32_t kRefIndexMask = 0x3FFFFFFFu;
+constexpr uint32_t kRefSphere = 0x00000000u;
+constexpr uint32_t kRefBox = 0x40000000u;
+constexpr uint32_t kRefAffineSphere = 0x80000000u;
+constexpr uint32_t kRefAffineBox = 0xC0000000u;
+constexpr int kKindSphere = 0, kKindBox = 1, kKindAffineSphere = 2, kKindAffineBox = 3, kKindPlane = 4;
+constexpr int kStatCount = 14;
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 v, float s) { return {v.x * s, v.y * s, v.z * s}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+// float3 / s is v * (1/s) in the reference (GpuKernelSources.java:87-90).
+__device__ __forceinline__ f3 divs(f3 v, float s) { float inv = 1.0f / s; return {v.x * inv, v.y * inv, v.z * inv}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 normalizeSafe(f3 v) { float len = sqrtf(fmaxf(dot(v, v), 1e-20f)); return divs(v, len); }
+__device__ __forceinline__ float comp(f3 v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
+
+// GpuKernelSources.java:101-110
+__device__ inline f3 normalizeTransformedNormal(f3 v) {
+	float l2 = dot(v, v);
+	if (l2 >= 1.0e-20f && isfinite(l2)) return divs(v, sqrtf(l2));
+	float scale = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+	if (!(scale > 0.0f) || !isfinite(scale)) return mk(0.0f, 0.0f, 0.0f);
+	f3 s = mk(v.x / scale, v.y / scale, v.z / scale);
+	return divs(s, sqrtf(dot(s, s)));
+}
+
+// triple32 hash and rand01 (GpuKernelSources.java:165-179).
+__device__ __forceinline__ uint32_t hash_u32(uint32_t x) {
+	x ^= x >> 17; x *= 0xed5ad4bbu;
+	x ^= x >> 11; x *= 0xac4c1b51u;
+	x ^= x >> 15; x *= 0x31848babu;
+	x ^= x >> 14;
+	return x;
+}
+__device__ __forceinline__ float rand01(uint32_t &s) {
+	s = hash_u32(s + 0x9e3779b9u);
+	return (float) (s >> 8) * 0x1p-24f;
+}
+// GpuKernelSources.java:181-183
+__device__ __forceinline__ float luminance(f3 c) { return c.x * 0.212655f + c.y * 0.715158f + c.z * 0.072187f; }
+
+// Single-precision sine and cosine after the Cephes library (S. L. Moshier,
+// sinf.c/cosf.c): octant reduction by a three-part pi/4 and its degree-7/8
+// minimax polynomials, in plain float arithmetic so host and device agree
+// bit for bit.  Arguments here lie in [0, 2 pi).
+__device__ inline void sincos_cephes(float xx, float &sOut, float &cOut) {
+	const float FOPI = 1.27323954473516f;
+	const float DP1 = 0.78515625f, DP2 = 2.4187564849853515625e-4f, DP3 = 3.77489497744594108e-8f;
+	float x = fabsf(xx);
+	int signS = xx < 0.0f ? -1 : 1, signC = 1;
+	int j = (int) (x * FOPI);
+	float y = (float) j;
+	if (j & 1) { j += 1; y += 1.0f; }
+	j &= 7;
+	if (j > 3) { j -= 4; signS = -signS; signC = -signC; }
+	if (j > 1) signC = -signC;
+	x = ((x - y * DP1) - y * DP2) - y * DP3;
+	float z = x * x;
+	float pc = 2.443315711809948e-5f * z - 1.388731625493765e-3f;
+	pc = pc * z + 4.166664568298827e-2f;
+	pc = pc * z * z - 0.5f * z + 1.0f;
+	float ps = -1.9515295891e-4f * z + 8.3321608736e-3f;
+	ps = ps * z - 1.6666654611e-1f;
+	ps = ps * z * x + x;
+	bool swap = (j == 1 || j == 2);
+	float s = swap ? pc : ps;
+	float c = swap ? ps : pc;
+	sOut = signS < 0 ? -s : s;
+	cOut = signC < 0 ? -c : c;
+}
+
+// Per-path seed (GpuKernelSources.java:1231-1253).
+__device__ __forceinline__ uint32_t path_seed(uint64_t frameSeed, int frameIndex, uint32_t pixel, int sampleOffset) {
+	uint32_t base = hash_u32((uint32_t) (frameSeed & 0xFFFFFFFFull));
+	base ^= hash_u32((uint32_t) (frameSeed >> 32));
+	base ^= hash_u32((uint32_t) frameIndex);
+	base ^= hash_u32(pixel);
+	base = hash_u32(base);
+	uint32_t seed = base ^ hash_u32((uint32_t) sampleOffset);
+	seed ^= hash_u32((uint32_t) sampleOffset * 0x9E3779B9u);
+	return hash_u32(seed);
+}
+
+// ------------------------------------------------------------ device scene
+
+struct MaterialF { f3 diffuse, reflective, refractive, emittance; float ior; };
+
+struct DevScene {
+	const float4 *nodes;        // 4 x float4 per node (see header comment)
+	const uint32_t *refs;
+	const float4 *spheres;      // {centre, radius}
+	const int *sphereMat;
+	const float4 *boxes;        // 2 per box: {lo.xyz, hi.x}, {hi.y, hi.z, mat bits, 0}
+	const float4 *affS;         // 3 per affine sphere: inverse rows with centre in .w
+	const float4 *affSN;        // 3 per affine sphere: normal matrix rows, material bits
+	const float4 *affB;
+	const float4 *affBN;
+	const float *planes;        // wire format, 26 floats per plane
+	const float4 *materials;    // 4 float4 per material: diffuse, reflective, refractive, emittance+ior
+	int sphereCount, boxCount, affSCount, affBCount, planeCount;
+	int nodeCount, refCount, rootCount, useBvh, materialCount;
+	int roots[4];
+	float rootBounds[4][6];
+	float bg[3];
+	int stackLimit;             // BVH_STACK_SIZE semantics
+};
+
+__device__ __forceinline__ MaterialF loadMaterial(const DevScene &S, int index) {
+	MaterialF m;
+	if (index < 0) {
+		m.diffuse = m.reflective = m.refractive = m.emittance = mk(0.0f, 0.0f, 0.0f);
+		m.ior = 0.0f;
+		return m;
+	}
+	const float4 *p = S.materials + 4 * index;
+	float4 a = p[0], b = p[1], c = p[2], d = p[3];
+	m.diffuse = mk(a.x, a.y, a.z);
+	m.reflective = mk(b.x, b.y, b.z);
+	m.refractive = mk(c.x, c.y, c.z);
+	m.emittance = mk(d.x, d.y, d.z);
+	m.ior = d.w;
+	return m;
+}
+
+// ------------------------------------------------------------ intersection
+// Each test returns the reference's "found" predicate against `limit`
+// (closestT in the reference) and writes the candidate distance.
+
+// traceSphere body (GpuKernelSources.java:343-381).
+struct SphereSolve { float t, radialOffset; f3 closest; };
+__device__ __forceinline__ bool sphereSolve(float4 s, f3 o, f3 d, SphereSolve &r) {
+	f3 oc = o - mk(s.x, s.y, s.z);
+	float a = dot(d, d);
+	float b = dot(oc, d);
+	float centerT = -b / a;
+	f3 cl = mk(fmaf(d.x, centerT, oc.x), fmaf(d.y, centerT, oc.y), fmaf(d.z, centerT, oc.z));
+	float corr = -dot(cl, d) / a;
+	centerT += corr;
+	cl = mk(fmaf(d.x, corr, cl.x), fmaf(d.y, corr, cl.y), fmaf(d.z, corr, cl.z));
+	float rs = fmaf(s.w, s.w, -dot(cl, cl));
+	if (rs <= 0.0f) return false;
+	float halfSpan = sqrtf(rs / a);
+	float t1 = centerT - halfSpan, t2 = centerT + halfSpan;
+	if (t1 > kEps) { r.t = t1; r.radialOffset = -halfSpan; }
+	else if (t2 > kEps) { r.t = t2; r.radialOffset = halfSpan; }
+	else return false;
+	r.closest = cl;
+	return true;
+}
+__device__ __forceinline__ bool testSphere(float4 s, f3 o, f3 d, float limit, float &t) {
+	SphereSolve r;
+	if (!sphereSolve(s, o, d, r)) return false;
+	t = r.t;
+	return r.t < limit;
+}
+
+// Slab of one axis; false when the zero-direction branch rejects
+// (GpuKernelSources.java:476-515, :599-639).
+__device__ __forceinline__ bool slab(float oc, float dc, float lo, float hi, float &tmin, float &tmax) {
+	if (dc == 0.0f) {
+		if (oc < lo || oc > hi) return false;
+		tmin = -INFINITY; tmax = INFINITY;
+		return true;
+	}
+	float t1 = (lo - oc) / dc, t2 = (hi - oc) / dc;
+	tmin = fminf(t1, t2); tmax = fmaxf(t1, t2);
+	return true;
+}
+
+// Enter/exit selection (GpuKernelSources.java:517-556, :641-680).
+struct BoxSolve { float t; int axis; float sign; };
+__device__ __forceinline__ bool boxSelect(float x0, float x1, float y0, float y1, float z0, float z1, float after, f3 d,
+                                          BoxSolve &r) {
+	float tEnter = fmaxf(fmaxf(x0, y0), z0);
+	float tExit = fminf(fminf(x1, y1), z1);
+	if (tEnter >= tExit || tExit <= after) return false;
+	if (tEnter > after) {
+		int axis = 0; float ev = x0;
+		if (y0 > ev) { ev = y0; axis = 1; }
+		if (z0 > ev) axis = 2;
+		r.t = tEnter; r.axis = axis;
+		r.sign = comp(d, axis) > 0.0f ? -1.0f : 1.0f;
+	} else {
+		int axis = 0; float xv = x1;
+		if (y1 < xv) { xv = y1; axis = 1; }
+		if (z1 < xv) axis = 2;
+		r.t = tExit; r.axis = axis;
+		r.sign = comp(d, axis) >= 0.0f ? 1.0f : -1.0f;
+	}
+	return true;
+}
+__device__ __forceinline__ f3 axisNormal(int axis, float sign) {
+	return axis == 0 ? mk(sign, 0.0f, 0.0f) : (axis == 1 ? mk(0.0f, sign, 0.0f) : mk(0.0f, 0.0f, sign));
+}
+
+// traceBox body (GpuKernelSources.java:594-684).
+__device__ __forceinline__ bool boxSolve(float4 b0, float4 b1, f3 o, f3 d, BoxSolve &r) {
+	float x0, x1, y0, y1, z0, z1;
+	if (!slab(o.x, d.x, b0.x, b0.w, x0, x1)) return false;
+	if (!slab(o.y, d.y, b0.y, b1.x, y0, y1)) return false;
+	if (!slab(o.z, d.z, b0.z, b1.y, z0, z1)) return false;
+	return boxSelect(x0, x1, y0, y1, z0, z1, kEps, d, r);
+}
+__device__ __forceinline__ bool testBox(float4 b0, float4 b1, f3 o, f3 d, float limit, float &t) {
+	BoxSolve r;
+	if (!boxSolve(b0, b1, o, d, r)) return false;
+	t = r.t;
+	return !(r.t >= limit);
+}
+
+// affineRayShift / applyAffine* (GpuKernelSources.java:276-301); rows hold
+// the inverse linear map with the world centre in .w.
+struct AffineLocal { float shift; f3 lo, ld; };
+__device__ __forceinline__ void affineLocal(float4 r0, float4 r1, float4 r2, f3 o, f3 d, AffineLocal &a) {
+	f3 c = mk(r0.w, r1.w, r2.w);
+	float den = dot(d, d);
+	a.shift = den > 0.0f ? fmaxf(0.0f, dot(c - o, d) / den) : 0.0f;
+	f3 rel = mk(fmaf(d.x, a.shift, o.x - r0.w), fmaf(d.y, a.shift, o.y - r1.w), fmaf(d.z, a.shift, o.z - r2.w));
+	a.lo = mk(r0.x * rel.x + r0.y * rel.y + r0.z * rel.z,
+	          r1.x * rel.x + r1.y * rel.y + r1.z * rel.z,
+	          r2.x * rel.x + r2.y * rel.y + r2.z * rel.z);
+	a.ld = mk(r0.x * d.x + r0.y * d.y + r0.z * d.z,
+	          r1.x * d.x + r1.y * d.y + r1.z * d.z,
+	          r2.x * d.x + r2.y * d.y + r2.z * d.z);
+}
+
+// normalizedDirection (GpuKernelSources.java:317-336).
+__device__ inline bool normalizedDirection(f3 v, f3 &unit, float &invLen) {
+	float l2 = dot(v, v);
+	if (isfinite(l2) && fabsf(l2 - 1.0f) <= 2.0e-6f) { unit = v; invLen = 1.0f; return true; }
+	float scale = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+	if (!(scale > 0.0f) || !isfinite(scale)) return false;
+	f3 s = mk(v.x / scale, v.y / scale, v.z / scale);
+	float sl = sqrtf(dot(s, s));
+	float dl = scale * sl;
+	float inv = 1.0f / dl;
+	if (!(inv > 0.0f) || !isfinite(inv)) return false;
+	unit = s * (1.0f / sl);
+	invLen = inv;
+	return true;
+}
+
+// traceAffineSphere body (GpuKernelSources.java:408-443); returns the local
+// parameter hitT and the world distance.
+__device__ __forceinline__ bool affineSphereSolve(const AffineLocal &a, float &hitT, float &worldT) {
+	float afterLocal = kEps - a.shift;
+	float solveA = dot(a.ld, a.ld);
+	f3 solveDir = a.ld;
+	float pscale = 1.0f;
+	if (!(solveA > 0.0f) || !isfinite(solveA)) {
+		f3 u; float inv;
+		if (!normalizedDirection(a.ld, u, inv)) return false;
+		solveDir = u; solveA = 1.0f; pscale = inv;
+	}
+	float scaledAfterT = afterLocal / pscale;
+	float centerT = -dot(a.lo, solveDir) / solveA;
+	f3 cl = a.lo + solveDir * centerT;
+	float rs = fmaf(-cl.x, cl.x, fmaf(-cl.y, cl.y, fmaf(-cl.z, cl.z, 1.0f)));
+	if (rs <= 0.0f) return false;
+	float halfSpan = sqrtf(rs / solveA);
+	float t1 = centerT - halfSpan, t2 = centerT + halfSpan;
+	float sh;
+	if (t1 > scaledAfterT) sh = t1;
+	else if (t2 > scaledAfterT) sh = t2;
+	else return false;
+	hitT = sh * pscale;
+	worldT = a.shift + hitT;
+	return true;
+}
+__device__ __forceinline__ bool testAffineSphere(const float4 *rec, f3 o, f3 d, float limit, float &t) {
+	AffineLocal a;
+	affineLocal(rec[0], rec[1], rec[2], o, d, a);
+	float hitT, worldT;
+	if (!affineSphereSolve(a, hitT, worldT)) return false;
+	t = worldT;
+	return !(worldT >= limit);
+}
+
+// traceAffineBox body (GpuKernelSources.java:470-558).
+__device__ __forceinline__ bool affineBoxSolve(const AffineLocal &a, BoxSolve &r, float &worldT) {
+	float x0, x1, y0, y1, z0, z1;
+	if (!slab(a.lo.x, a.ld.x, -1.0f, 1.0f, x0, x1)) return false;
+	if (!slab(a.lo.y, a.ld.y, -1.0f, 1.0f, y0, y1)) return false;
+	if (!slab(a.lo.z, a.ld.z, -1.0f, 1.0f, z0, z1)) return false;
+	if (!boxSelect(x0, x1, y0, y1, z0, z1, kEps - a.shift, a.ld, r)) return false;
+	worldT = a.shift + r.t;
+	return true;
+}
+__device__ __forceinline__ bool testAffineBox(const float4 *rec, f3 o, f3 d, float limit, float &t) {
+	AffineLocal a;
+	affineLocal(rec[0], rec[1], rec[2], o, d, a);
+	BoxSolve r;
+	float worldT;
+	if (!affineBoxSolve(a, r, worldT)) return false;
+	t = worldT;
+	return !(worldT >= limit);
+}
+
+// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.
+__device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 d,
+                                               float beforeT) {
+	float tMin = kEps;
+	float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
+	float tMax = cb;
+#define MRAD_ENTRY_AXIS(OC, DC, LO, HI)                        \
+	if (DC == 0.0f) {                                          \
+		if (OC < LO || OC > HI) return INFINITY;               \
+	} else {                                                   \
+		float t0 = (LO - OC) / DC, t1 = (HI - OC) / DC;        \
+		if (t0 > t1) { float tmp = t0; t0 = t1; t1 = tmp; }    \
+		if (t1 > 0.0f) t1 *= 1.00000072f;                      \
+		tMin = fmaxf(tMin, t0);                                \
+		tMax = fminf(tMax, t1);                                \
+		if (tMax < tMin) return INFINITY;                      \
+	}
+	MRAD_ENTRY_AXIS(o.x, d.x, lx, hx)
+	MRAD_ENTRY_AXIS(o.y, d.y, ly, hy)
+	MRAD_ENTRY_AXIS(o.z, d.z, lz, hz)
+#undef MRAD_ENTRY_AXIS
+	return tMin <= cb ? tMin : INFINITY;
+}
+
+// ------------------------------------------------------------ closest hit
+
+struct Best { float t; int order; int kind; int index; };
+
+template <bool kMetrics>
+struct Counters {
+	uint32_t v[kStatCount];
+	__device__ __forceinline__ void zero() {
+#pragma unroll
+		for (int i = 0; i < kStatCount; i++) v[i] = 0;
+	}
+	__device__ __forceinline__ void add(int slot, uint32_t n) { v[slot] += n; }
+	__device__ __forceinline__ void maxStack(int size) { v[9] = v[9] > (uint32_t) size ? v[9] : (uint32_t) size; }
+};
+template <>
+struct Counters<false> {
+	__device__ __forceinline__ void zero() {}
+	__device__ __forceinline__ void add(int, uint32_t) {}
+	__device__ __forceinline__ void maxStack(int) {}
+};
+
+__device__ __forceinline__ int typeSlot(uint32_t type) {
+	return type == kRefBox ? 1 : type == kRefAffineSphere ? 3 : type == kRefAffineBox ? 4 : 0;
+}
+
+// tracePrimitiveRef with the reference's (t, order) acceptance
+// (GpuKernelSources.java:772-806).
+__device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_t type, f3 o, f3 d, Best &best) {
+	const int index = (int) (ref & kRefIndexMask);
+	const float limit = nextafterf(best.t, INFINITY);
+	float t = 0.0f;
+	bool found;
+	int order, kind;
+	if (type == kRefBox) {
+		order = S.sphereCount + index; kind = kKindBox;
+		found = testBox(S.boxes[2 * index], S.boxes[2 * index + 1], o, d, limit, t);
+	} else if (type == kRefAffineSphere) {
+		order = S.sphereCount + S.boxCount + index; kind = kKindAffineSphere;
+		found = testAffineSphere(S.affS + 3 * index, o, d, limit, t);
+	} else if (type == kRefAffineBox) {
+		order = S.sphereCount + S.boxCount + S.affSCount + index; kind = kKindAffineBox;
+		found = testAffineBox(S.affB + 3 * index, o, d, limit, t);
+	} else {
+		order = index; kind = kKindSphere;
+		found = testSphere(S.spheres[index], o, d, limit, t);
+	}
+	if (!found || t > best.t || (t == best.t && order >= best.order)) return;
+	best.t = t; best.order = order; best.kind = kind; best.index = index;
+}
+
+// traceBvh (GpuKernelSources.java:844-961): same stack discipline, same
+// counters.  The stack lives in thread-private memory sized by kCap; the
+// runtime limit S.stackLimit (<= kCap) gives BVH_STACK_SIZE semantics.
+template <bool kMetrics, int kCap>
+__device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c) {
+	if (S.nodeCount <= 0 || S.refCount <= 0 || S.rootCount <= 0 || S.rootCount > 4) return;
+	int nodeStack[kCap];
+	float entryStack[kCap];
+	const int cap = S.stackLimit;
+	int size = 0;
+	bool overflow = false;
+	for (int r = 0; r < S.rootCount; r++) {
+		int ri = S.roots[r];
+		if (ri < 0 || ri >= S.nodeCount) { overflow = true; break; }
+		c.add(1, 1);
+		const float *rb = S.rootBounds[r];
+		float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, d, best.t);
+		if (!isfinite(e)) continue;
+		if (size >= cap) { overflow = true; break; }
+		int pos = size;
+		while (pos > 0 && e >= entryStack[pos - 1]) {
+			nodeStack[pos] = nodeStack[pos - 1];
+			entryStack[pos] = entryStack[pos - 1];
+			pos--;
+		}
+		nodeStack[pos] = ri;
+		entryStack[pos] = e;
+		size++;
+		c.maxStack(size);
+	}
+	while (!overflow && size > 0) {
+		--size;
+		const int node = nodeStack[size];
+		const float entry = entryStack[size];
+		const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
+		if (entry > ch) continue;
+		const float4 *rec = S.nodes + 4 * (size_t) node;
+		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
+		if (nd.x < 0) {
+			c.add(11, 1);
+			c.add(nd.x == -1 ? 13 : 12, 1);
+			c.add(2, (uint32_t) nd.w);
+			const int kind = nd.x;
+			const uint32_t leafType = kind == -2 ? kRefSphere : kind == -3 ? kRefBox
+			                        : kind == -4 ? kRefAffineSphere : kRefAffineBox;
+			for (int i = 0; i < nd.w; i++) {
+				const int ri = nd.z + i;
+				if (ri < 0 || ri >= S.refCount) continue;
+				const uint32_t ref = S.refs[ri];
+				const uint32_t type = kind == -1 ? (ref & kRefTypeMask) : leafType;
+				c.add(3 + typeSlot(type), 1);
+				testRef(S, ref, type, o, d, best);
+			}
+			continue;
+		}
+		c.add(10, 1);
+		const float4 a = rec[0], b = rec[1], cc = rec[2];
+		const float le = entryDistance(a.x, a.y, a.z, a.w, b.x, b.y, o, d, best.t);
+		const float re = entryDistance(b.z, b.w, cc.x, cc.y, cc.z, cc.w, o, d, best.t);
+		c.add(1, 2);
+		int nearN = nd.x, farN = nd.y;
+		float nearE = le, farE = re;
+		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+		if (isfinite(farE)) {
+			if (size >= cap) { overflow = true; break; }
+			nodeStack[size] = farN; entryStack[size] = farE; size++;
+			c.maxStack(size);
+		}
+		if (isfinite(nearE)) {
+			if (size >= cap) { overflow = true; break; }
+			nodeStack[size] = nearN; entryStack[size] = nearE; size++;
+			c.maxStack(size);
+		}
+	}
+	if (overflow) {
+		c.add(8, 1);
+		for (int i = 0; i < S.refCount; i++) {
+			const uint32_t ref = S.refs[i];
+			c.add(2, 1);
+			c.add(3 + typeSlot(ref & kRefTypeMask), 1);
+			testRef(S, ref, ref & kRefTypeMask, o, d, best);
+		}
+	}
+}
+
+// Linear scan of every primitive class in global order, strict against the
+// running best (GpuKernelSources.java:993-1013).
+template <bool kMetrics>
+__device__ inline void traceLinear(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c) {
+	c.add(2, (uint32_t) (S.sphereCount + S.boxCount + S.affSCount + S.affBCount));
+	c.add(3, (uint32_t) S.sphereCount);
+	c.add(4, (uint32_t) S.boxCount);
+	c.add(6, (uint32_t) S.affSCount);
+	c.add(7, (uint32_t) S.affBCount);
+	float t;
+	for (int i = 0; i < S.sphereCount; i++)
+		if (testSphere(S.spheres[i], o, d, best.t, t)) best = {t, i, kKindSphere, i};
+	for (int i = 0; i < S.boxCount; i++)
+		if (testBox(S.boxes[2 * i], S.boxes[2 * i + 1], o, d, best.t, t)) best = {t, S.sphereCount + i, kKindBox, i};
+	for (int i = 0; i < S.affSCount; i++)
+		if (testAffineSphere(S.affS + 3 * i, o, d, best.t, t)) best = {t, S.sphereCount + S.boxCount + i, kKindAffineSphere, i};
+	for (int i = 0; i < S.affBCount; i++)
+		if (testAffineBox(S.affB + 3 * i, o, d, best.t, t))
+			best = {t, S.sphereCount + S.boxCount + S.affSCount + i, kKindAffineBox, i};
+}
+
+// tracePlane (GpuKernelSources.java:708-734) distance part.
+__device__ inline void tracePlanes(const DevScene &S, f3 o, f3 d, Best &best) {
+	const int base = S.sphereCount + S.boxCount + S.affSCount + S.affBCount;
+	for (int i = 0; i < S.planeCount; i++) {
+		const float *p = S.planes + 26 * i;
+		f3 n = mk(p[0], p[1], p[2]);
+		float denom = dot(n, d);
+		if (denom == 0.0f) continue;
+		float t = -(dot(n, o) + p[3]) / denom;
+		if (t <= kEps || t >= best.t) continue;
+		best = {t, base + i, kKindPlane, i};
+	}
+}
+
+// choosePlaneMaterialIndex (GpuKernelSources.java:251-269).
+__device__ inline int planeMaterialIndex(const float *p, f3 hp) {
+	const float *g = p + 18;
+	const int surface = (int) g[5];
+	if (g[0] == 0.0f) return surface;
+	f3 rel = hp - mk(p[4], p[5], p[6]);
+	float b_e = dot(rel, mk(p[7], p[8], p[9])) / p[13];
+	float b_f = dot(rel, mk(p[10], p[11], p[12])) / p[14];
+	float u = (b_e - b_f * p[16]) / p[17];
+	float v = (b_f - b_e * p[15]) / p[17];
+	float hx = g[3] * 0.5f, hy = g[4] * 0.5f;
+	float mx = (u + hx) - g[1] * floorf((u + hx) / g[1]);
+	float my = (v + hy) - g[2] * floorf((v + hy) / g[2]);
+	if (mx < g[3] || my < g[4]) return (int) g[7];
+	return (int) g[6];
+}
+
+struct HitDetail { f3 pos, normal; int matIndex; };
+
+// Normal, position and material of the winning primitive, recomputed with
+// the arithmetic of the routine that accepted it.
+__device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, const Best &best) {
+	HitDetail h;
+	h.pos = o + d * best.t;
+	const int i = best.index;
+	if (best.kind == kKindSphere) {
+		SphereSolve r;
+		sphereSolve(S.spheres[i], o, d, r);
+		h.normal = normalizeSafe(mk(fmaf(d.x, r.radialOffset, r.closest.x), fmaf(d.y, r.radialOffset, r.closest.y),
+		                            fmaf(d.z, r.radialOffset, r.closest.z)));
+		h.matIndex = S.sphereMat[i];
+	} else if (best.kind == kKindBox) {
+		float4 b1 = S.boxes[2 * i + 1];
+		BoxSolve r;
+		boxSolve(S.boxes[2 * i], b1, o, d, r);
+		h.normal = axisNormal(r.axis, r.sign);
+		h.matIndex = __float_as_int(b1.z);
+	} else if (best.kind == kKindAffineSphere || best.kind == kKindAffineBox) {
+		const bool isBox = best.kind == kKindAffineBox;
+		const float4 *rec = (isBox ? S.affB : S.affS) + 3 * i;
+		const float4 *nrm = (isBox ? S.affBN : S.affSN) + 3 * i;
+		AffineLocal a;
+		affineLocal(rec[0], rec[1], rec[2], o, d, a);
+		f3 local;
+		if (isBox) {
+			BoxSolve r;
+			float worldT;
+			affineBoxSolve(a, r, worldT);
+			local = axisNormal(r.axis, r.sign);
+		} else {
+			float hitT, worldT;
+			affineSphereSolve(a, hitT, worldT);
+			local = a.lo + a.ld * hitT;
+		}
+		float4 n0 = nrm[0], n1 = nrm[1], n2 = nrm[2];
+		f3 wn = mk(n0.x * local.x + n0.y * local.y + n0.z * local.z,
+		           n0.w * local.x + n1.x * local.y + n1.y * local.z,
+		           n1.z * local.x + n1.w * local.y + n2.x * local.z);
+		h.normal = normalizeTransformedNormal(wn);
+		h.matIndex = __float_as_int(n2.y);
+	} else {
+		const float *p = S.planes + 26 * i;
+		h.normal = mk(p[0], p[1], p[2]);
+		h.matIndex = planeMaterialIndex(p, h.pos);
+	}
+	return h;
+}
+
+// traceClosest (GpuKernelSources.java:963-1020).
+template <bool kMetrics, int kCap>
+__device__ __forceinline__ bool traceClosest(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c) {
+	best.t = 1e30f;
+	best.order = 0x7FFFFFFF;
+	best.kind = -1;
+	best.index = -1;
+	if (S.useBvh != 0 && S.nodeCount > 0) traceBvh<kMetrics, kCap>(S, o, d, best, c);
+	else traceLinear<kMetrics>(S, o, d, best, c);
+	c.add(2, (uint32_t) S.planeCount);
+	c.add(5, (uint32_t) S.planeCount);
+	tracePlanes(S, o, d, best);
+	return best.kind >= 0;
+}
+
+// reflect_dir / refract_dir / sampleUnitSphere (GpuKernelSources.java:1022-1059).
+__device__ __forceinline__ f3 reflectDir(f3 n, f3 in) { return n * (2.0f * dot(in, n)) - in; }
+__device__ inline f3 refractDir(f3 incoming, f3 n, float eta) {
+	f3 i = normalizeSafe(incoming);
+	float c1 = dot(i, n);
+	float ri = (c1 >= 0.0f) ? eta : -1.0f / eta;
+	float invRi2 = 1.0f / (ri * ri);
+	float c2 = 1.0f - (1.0f - c1 * c1) * invRi2;
+	if (c2 <= 0.0f) return normalizeSafe(reflectDir(n, i));
+	float sq = sqrtf(c2);
+	return normalizeSafe(n * (c1 - sq * ri) - i);
+}
+__device__ __forceinline__ f3 sampleUnitSphere(uint32_t &rng) {
+	float z = rand01(rng) * 2.0f - 1.0f;
+	float t = rand01(rng) * 2.0f * 3.141592654f;
+	float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+	float sn, cs;
+	sincos_cephes(t, sn, cs);
+	return mk(r * cs, r * sn, z);
+}
+
+// One scattering event of tracePath (GpuKernelSources.java:1120-1161);
+// returns false when the path is absorbed (all lobe weights zero).
+__device__ __forceinline__ bool scatter(const MaterialF &m, f3 normal, f3 &dir, uint32_t &rng, f3 &weight) {
+	f3 in = -dir;
+	float wD = luminance(m.diffuse), wR = luminance(m.reflective), wT = luminance(m.refractive);
+	float sum = wD + wR + wT;
+	if (sum <= 0.0f) return false;
+	float xi = rand01(rng) * sum;
+	if (xi < wD) {
+		float p = wD / sum;
+		dir = sampleUnitSphere(rng) + normal;
+		weight = divs(m.diffuse, p);
+	} else if (xi < wD + wR) {
+		float p = wR / sum;
+		dir = reflectDir(normal, in);
+		weight = divs(m.reflective, p);
+	} else {
+		float p = wT / sum;
+		dir = refractDir(in, normal, m.ior);
+		weight = divs(m.refractive, p);
+	}
+	return true;
+}
+
+// Russian roulette after bounce rrStart (no counterpart in the reference).
+__device__ __forceinline__ bool roulette(int depth, int maxDepth, int rrStart, f3 &T, uint32_t &rng) {
+	if (rrStart < 0 || depth + 1 < rrStart || depth + 1 >= maxDepth) return true;
+	float m = fmaxf(T.x, fmaxf(T.y, T.z));
+	float q = fminf(fmaxf(m, 0.05f), 0.95f);
+	if (rand01(rng) >= q) return false;
+	T = T * (1.0f / q);
+	return true;
+}
+
+// tracePath (GpuKernelSources.java:1061-1165).
+template <bool kMetrics, int kCap>
+__device__ inline f3 tracePath(const DevScene &S, f3 o, f3 d, int maxDepth, int rrStart, uint32_t &rng,
+                               Counters<kMetrics> &c) {
+	f3 T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
+	const f3 bg = mk(S.bg[0], S.bg[1], S.bg[2]);
+	for (int depth = 0; depth < maxDepth; ++depth) {
+		c.add(0, 1);
+		Best best;
+		if (!traceClosest<kMetrics, kCap>(S, o, d, best, c)) {
+			L = L + T * bg;
+			break;
+		}
+		HitDetail h = resolveHit(S, o, d, best);
+		MaterialF m = loadMaterial(S, h.matIndex);
+		L = L + T * m.emittance;
+		f3 w;
+		if (!scatter(m, h.normal, d, rng, w)) break;
+		o = h.pos + d * kEps;
+		T = T * w;
+		if (!roulette(depth, maxDepth, rrStart, T, rng)) break;
+	}
+	return L;
+}
+
+}  // namespace mrad

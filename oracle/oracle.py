@@ -1,0 +1,171 @@
+"""ctypes face of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (master-rad_amd/)
+never imports it.
+
+Scenes are passed in the reference's packed wire format (GpuScene.java:24-31)
+as produced by mrad.GpuScene.wire().
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+STAT_COUNT = 14
+
+fp = C.POINTER(C.c_float)
+ip = C.POINTER(C.c_int32)
+
+
+class OrScene(C.Structure):
+    _fields_ = [("spheres", fp), ("sphereCount", C.c_int), ("boxes", fp), ("boxCount", C.c_int),
+                ("planes", fp), ("planeCount", C.c_int), ("affineSpheres", fp), ("affineSphereCount", C.c_int),
+                ("affineBoxes", fp), ("affineBoxCount", C.c_int), ("nodeBounds", fp), ("nodeCount", C.c_int),
+                ("nodeData", ip), ("refs", ip), ("refCount", C.c_int), ("roots", ip), ("rootCount", C.c_int),
+                ("useBvh", C.c_int), ("materials", fp), ("materialCount", C.c_int), ("bg", C.c_float * 3),
+                ("stackSize", C.c_int)]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def _lib() -> C.CDLL:
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    lib.or_hash_u32.restype = C.c_uint32
+    lib.or_hash_u32.argtypes = [C.c_uint32]
+    lib.or_rand01.restype = C.c_float
+    lib.or_rand01.argtypes = [C.POINTER(C.c_uint32)]
+    lib.or_sincosf.restype = None
+    lib.or_sincosf.argtypes = [C.c_float, fp, fp]
+    lib.or_path_seed.restype = C.c_uint32
+    lib.or_path_seed.argtypes = [C.c_uint64, C.c_int, C.c_uint32, C.c_int]
+    lib.or_render_pixels.restype = C.c_int
+    lib.or_render_pixels.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_uint64, C.c_int, ip, C.c_int, fp, C.POINTER(C.c_uint64), C.c_int]
+    lib.or_trace_replay.restype = C.c_int
+    lib.or_trace_replay.argtypes = [C.POINTER(OrScene), fp, C.c_int, ip, fp, ip, ip, fp, C.POINTER(C.c_uint32)]
+    lib.or64_render.restype = C.c_int
+    lib.or64_render.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    lib.or64_closest.restype = C.c_int
+    lib.or64_closest.argtypes = [C.POINTER(OrScene), fp, C.POINTER(C.c_double), ip, ip, C.POINTER(C.c_double)]
+    return lib
+
+
+lib = _lib()
+
+
+class Scene:
+    """Holds numpy copies of the wire arrays and the C view of them."""
+
+    def __init__(self, wire: dict, use_bvh: bool = True, stack_size: int = 32):
+        self.w = {k: np.ascontiguousarray(v) for k, v in wire.items()}
+        w = self.w
+
+        def f(name):
+            a = w[name]
+            return a.ctypes.data_as(fp) if a.size else None
+
+        def i(name):
+            a = w[name]
+            return a.ctypes.data_as(ip) if a.size else None
+
+        s = OrScene()
+        s.spheres, s.sphereCount = f("spheres"), len(w["spheres"])
+        s.boxes, s.boxCount = f("boxes"), len(w["boxes"])
+        s.planes, s.planeCount = f("planes"), len(w["planes"])
+        s.affineSpheres, s.affineSphereCount = f("affine_spheres"), len(w["affine_spheres"])
+        s.affineBoxes, s.affineBoxCount = f("affine_boxes"), len(w["affine_boxes"])
+        s.nodeBounds, s.nodeCount = f("node_bounds"), (len(w["node_bounds"]) if use_bvh else 0)
+        s.nodeData = i("node_data")
+        s.refs, s.refCount = i("refs"), (len(w["refs"]) if use_bvh else 0)
+        s.roots, s.rootCount = i("roots"), (len(w["roots"]) if use_bvh else 0)
+        s.useBvh = 1 if use_bvh else 0
+        s.materials, s.materialCount = f("materials"), len(w["materials"])
+        for k in range(3):
+            s.bg[k] = float(w["background"][k])
+        s.stackSize = stack_size
+        self.c = s
+
+
+def hash_u32(x: int) -> int:
+    return lib.or_hash_u32(x)
+
+
+def sincosf(x: float) -> tuple[float, float]:
+    s, c = C.c_float(), C.c_float()
+    lib.or_sincosf(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def path_seed(frame_seed: int, frame_index: int, pixel: int, sample: int) -> int:
+    return lib.or_path_seed(frame_seed, frame_index, pixel, sample)
+
+
+def trace_replay(scene: Scene, rays: np.ndarray):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1)
+    n = len(rays) // 6
+    flags = np.zeros(n, np.int32)
+    dist = np.zeros(n, np.float32)
+    order = np.zeros(n, np.int32)
+    mat = np.zeros(n, np.int32)
+    normals = np.zeros(3 * n, np.float32)
+    per = np.zeros((n, STAT_COUNT), np.uint32)
+    rc = lib.or_trace_replay(C.byref(scene.c), rays.ctypes.data_as(fp), n, flags.ctypes.data_as(ip),
+                             dist.ctypes.data_as(fp), order.ctypes.data_as(ip), mat.ctypes.data_as(ip),
+                             normals.ctypes.data_as(fp), per.ctypes.data_as(C.POINTER(C.c_uint32)))
+    assert rc == 0
+    return flags, dist, order, mat, normals, per
+
+
+def render_pixels(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, frame_index: int,
+                  frame_seed: int, spp: int, pixels=None, rr_start: int = -1, threads: int | None = None):
+    """Float image (n, 3) of the listed pixels (all when None) and the 14 frame counters."""
+    cam = np.ascontiguousarray(camera, np.float32)
+    if pixels is not None:
+        pixels = np.ascontiguousarray(pixels, np.int32)
+        n = len(pixels)
+    else:
+        n = width * height
+    out = np.zeros((n, 3), np.float32)
+    stats = (C.c_uint64 * STAT_COUNT)()
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.or_render_pixels(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, max_depth, rr_start,
+                              frame_index, frame_seed & ((1 << 64) - 1), spp,
+                              pixels.ctypes.data_as(ip) if pixels is not None else None, n,
+                              out.ctypes.data_as(fp), stats, threads)
+    assert rc == 0
+    return out, np.array(stats[:], dtype=np.uint64)
+
+
+def render64(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, spp: int, seed: int,
+             rr_start: int = -1, threads: int | None = None, variance: bool = False):
+    cam = np.ascontiguousarray(camera, np.float32)
+    mean = np.zeros((height * width, 3), np.float64)
+    var = np.zeros((height * width, 3), np.float64) if variance else None
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.or64_render(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, max_depth, rr_start, spp,
+                         seed & ((1 << 64) - 1), mean.ctypes.data_as(C.POINTER(C.c_double)),
+                         var.ctypes.data_as(C.POINTER(C.c_double)) if variance else None, threads)
+    assert rc == 0
+    return (mean, var) if variance else mean
+
+
+def closest64(scene: Scene, ray6):
+    r = np.ascontiguousarray(ray6, np.float32)
+    t = C.c_double()
+    kind, index = C.c_int32(), C.c_int32()
+    n = (C.c_double * 3)()
+    f = lib.or64_closest(C.byref(scene.c), r.ctypes.data_as(fp), C.byref(t), C.byref(kind), C.byref(index), n)
+    return bool(f), t.value, kind.value, index.value, np.array(n[:])
