@@ -52,8 +52,11 @@ def code_object_sha256() -> str:
     return h.hexdigest()
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, out: Path | None = None, defines: tuple[str, ...] = ()) -> Path:
+    """Builds the library (to `out`, default master-rad_amd/mrad/libmrad.so);
+    `defines` are extra -D tuning macros of the kernels."""
     BUILD.mkdir(exist_ok=True)
+    lib = out or LIB
     objs = []
     for src in CPP_SOURCES:
         obj = BUILD / (Path(src).stem + ".o")
@@ -62,13 +65,14 @@ def build(verbose: bool = True) -> Path:
     sha = code_object_sha256()
     for src in HIP_SOURCES:
         obj = BUILD / (Path(src).stem + ".o")
-        run([HIPCC, *DEVICE_FLAGS, "-fPIC", f'-DMRAD_CODE_OBJECT_SHA256="{sha}"', "-I", str(ROOT / "include"),
+        run([HIPCC, *DEVICE_FLAGS, *(f"-D{d}" for d in defines), "-fPIC", f'-DMRAD_CODE_OBJECT_SHA256="{sha}"',
+             "-I", str(ROOT / "include"),
              "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -403,14 +403,47 @@ __device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_
 	best.t = t; best.order = order; best.kind = kind; best.index = index;
 }
 
+// Traversal stack storage.  The top kShortStack entries of each lane live in
+// LDS, lane-interleaved (entry i of lane l at [i * 64 + l]) so that a wave's
+// accesses hit 64 distinct banks whatever each lane's depth; deeper entries
+// spill to a thread-private array.  Capacity and overflow semantics are the
+// reference's BVH_STACK_SIZE (GpuKernelSources.java:858-859, :871, :924, :933).
+#ifndef MRAD_SHORT_STACK
+#define MRAD_SHORT_STACK 8
+#endif
+constexpr int kShortStack = MRAD_SHORT_STACK;
+
+struct LdsStack {
+	int *node;     // lane base, stride 64
+	float *entry;
+};
+
+template <int kCap>
+struct TravStack {
+	static constexpr int kSpill = kCap > kShortStack ? kCap - kShortStack : 1;
+	LdsStack lds;
+	int spillNode[kSpill];
+	float spillEntry[kSpill];
+	__device__ __forceinline__ void set(int i, int n, float e) {
+		if (kShortStack > 0 && i < kShortStack) { lds.node[i * 64] = n; lds.entry[i * 64] = e; }
+		else { spillNode[i - kShortStack] = n; spillEntry[i - kShortStack] = e; }
+	}
+	__device__ __forceinline__ int node(int i) const {
+		return (kShortStack > 0 && i < kShortStack) ? lds.node[i * 64] : spillNode[i - kShortStack];
+	}
+	__device__ __forceinline__ float entry(int i) const {
+		return (kShortStack > 0 && i < kShortStack) ? lds.entry[i * 64] : spillEntry[i - kShortStack];
+	}
+};
+
 // traceBvh (GpuKernelSources.java:844-961): same stack discipline, same
-// counters.  The stack lives in thread-private memory sized by kCap; the
-// runtime limit S.stackLimit (<= kCap) gives BVH_STACK_SIZE semantics.
+// counters.  kCap is the storage capacity; the runtime limit S.stackLimit
+// (<= kCap) gives BVH_STACK_SIZE semantics.
 template <bool kMetrics, int kCap>
-__device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c) {
+__device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c, LdsStack lds) {
 	if (S.nodeCount <= 0 || S.refCount <= 0 || S.rootCount <= 0 || S.rootCount > 4) return;
-	int nodeStack[kCap];
-	float entryStack[kCap];
+	TravStack<kCap> st;
+	st.lds = lds;
 	const int cap = S.stackLimit;
 	int size = 0;
 	bool overflow = false;
@@ -423,20 +456,18 @@ __device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Count
 		if (!isfinite(e)) continue;
 		if (size >= cap) { overflow = true; break; }
 		int pos = size;
-		while (pos > 0 && e >= entryStack[pos - 1]) {
-			nodeStack[pos] = nodeStack[pos - 1];
-			entryStack[pos] = entryStack[pos - 1];
+		while (pos > 0 && e >= st.entry(pos - 1)) {
+			st.set(pos, st.node(pos - 1), st.entry(pos - 1));
 			pos--;
 		}
-		nodeStack[pos] = ri;
-		entryStack[pos] = e;
+		st.set(pos, ri, e);
 		size++;
 		c.maxStack(size);
 	}
 	while (!overflow && size > 0) {
 		--size;
-		const int node = nodeStack[size];
-		const float entry = entryStack[size];
+		const int node = st.node(size);
+		const float entry = st.entry(size);
 		const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 		if (entry > ch) continue;
 		const float4 *rec = S.nodes + 4 * (size_t) node;
@@ -446,13 +477,14 @@ __device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Count
 			c.add(nd.x == -1 ? 13 : 12, 1);
 			c.add(2, (uint32_t) nd.w);
 			const int kind = nd.x;
+			const bool mixed = kind > -2 || kind < -5;
 			const uint32_t leafType = kind == -2 ? kRefSphere : kind == -3 ? kRefBox
 			                        : kind == -4 ? kRefAffineSphere : kRefAffineBox;
 			for (int i = 0; i < nd.w; i++) {
 				const int ri = nd.z + i;
 				if (ri < 0 || ri >= S.refCount) continue;
 				const uint32_t ref = S.refs[ri];
-				const uint32_t type = kind == -1 ? (ref & kRefTypeMask) : leafType;
+				const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
 				c.add(3 + typeSlot(type), 1);
 				testRef(S, ref, type, o, d, best);
 			}
@@ -468,12 +500,14 @@ __device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Count
 		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
 		if (isfinite(farE)) {
 			if (size >= cap) { overflow = true; break; }
-			nodeStack[size] = farN; entryStack[size] = farE; size++;
+			st.set(size, farN, farE);
+			size++;
 			c.maxStack(size);
 		}
 		if (isfinite(nearE)) {
 			if (size >= cap) { overflow = true; break; }
-			nodeStack[size] = nearN; entryStack[size] = nearE; size++;
+			st.set(size, nearN, nearE);
+			size++;
 			c.maxStack(size);
 		}
 	}
@@ -593,12 +627,13 @@ __device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, const Best
 
 // traceClosest (GpuKernelSources.java:963-1020).
 template <bool kMetrics, int kCap>
-__device__ __forceinline__ bool traceClosest(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c) {
+__device__ __forceinline__ bool traceClosest(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c,
+                                             LdsStack lds) {
 	best.t = 1e30f;
 	best.order = 0x7FFFFFFF;
 	best.kind = -1;
 	best.index = -1;
-	if (S.useBvh != 0 && S.nodeCount > 0) traceBvh<kMetrics, kCap>(S, o, d, best, c);
+	if (S.useBvh != 0 && S.nodeCount > 0) traceBvh<kMetrics, kCap>(S, o, d, best, c, lds);
 	else traceLinear<kMetrics>(S, o, d, best, c);
 	c.add(2, (uint32_t) S.planeCount);
 	c.add(5, (uint32_t) S.planeCount);
@@ -664,13 +699,13 @@ __device__ __forceinline__ bool roulette(int depth, int maxDepth, int rrStart, f
 // tracePath (GpuKernelSources.java:1061-1165).
 template <bool kMetrics, int kCap>
 __device__ inline f3 tracePath(const DevScene &S, f3 o, f3 d, int maxDepth, int rrStart, uint32_t &rng,
-                               Counters<kMetrics> &c) {
+                               Counters<kMetrics> &c, LdsStack lds) {
 	f3 T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
 	const f3 bg = mk(S.bg[0], S.bg[1], S.bg[2]);
 	for (int depth = 0; depth < maxDepth; ++depth) {
 		c.add(0, 1);
 		Best best;
-		if (!traceClosest<kMetrics, kCap>(S, o, d, best, c)) {
+		if (!traceClosest<kMetrics, kCap>(S, o, d, best, c, lds)) {
 			L = L + T * bg;
 			break;
 		}

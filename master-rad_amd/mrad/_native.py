@@ -9,7 +9,10 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libmrad.so"
+import os
+
+# MRAD_LIBRARY selects another build of the same library (e.g. a tuned variant).
+LIB_PATH = Path(os.environ.get("MRAD_LIBRARY", Path(__file__).resolve().parent / "libmrad.so"))
 
 MRAD_OK = 0
 MRAD_ERR_INVALID_ARGUMENT = -1

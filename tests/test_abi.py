@@ -1,0 +1,83 @@
+"""The C-ABI library: symbols, error mapping and host-side validation (CPU only;
+no compute call needs a GPU here)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mrad
+from mrad import _native as N
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "mrad.h").read_text()
+    return sorted(set(re.findall(r"\b(mrad_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(str(N.LIB_PATH))
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(N.SIGNATURES), "binding and header disagree"
+
+
+def test_validate_full_path_depth():
+    mrad.validate_full_path_depth(0)
+    mrad.validate_full_path_depth(32)
+    for bad in (-1, 33):
+        with pytest.raises(mrad.InvalidArgument, match=r"\[0, 32\]"):
+            mrad.validate_full_path_depth(bad)
+
+
+def test_code_object_hash_is_sha256():
+    h = mrad.GpuRayTracer.compiled_code_object_sha256()
+    assert re.fullmatch(r"[0-9a-f]{64}", h)
+
+
+def test_invalid_stack_size_rejected():
+    for bad in (4, 257):
+        with pytest.raises(mrad.InvalidArgument):
+            mrad.GpuRayTracer(8, 8, 1, True, bvh_stack_size=bad)
+
+
+def test_tracer_without_device_is_unavailable_and_errors_are_illegal_state(monkeypatch):
+    if mrad.GpuRayTracer(1, 1, 1, True).is_available():
+        pytest.skip("a HIP device is present; the unavailable path is exercised on CPU hosts")
+    t = mrad.GpuRayTracer(4, 4, 1, True)
+    assert not t.is_available()
+    sc = mrad.GpuScene()
+    sc.add_spheres([[0, 0, 0, 1]], mrad.material_array((0.5, 0.5, 0.5)))
+    cam = mrad.GpuCamera.look_at((0, 0, -3), (0, 0, 0), 40)
+    with pytest.raises(mrad.IllegalState, match="unavailable"):
+        t.render_sample(np.zeros(48, np.float32), sc, cam, 4, 0, 1)
+    with pytest.raises(mrad.IllegalState):
+        t.trace_replay(sc, np.zeros(6, np.float32))
+
+
+def test_argument_checks_precede_device_checks():
+    t = mrad.GpuRayTracer(4, 4, 1, True)
+    sc = mrad.GpuScene()
+    sc.add_spheres([[0, 0, 0, 1]], mrad.material_array((0.5, 0.5, 0.5)))
+    cam = mrad.GpuCamera.look_at((0, 0, -3), (0, 0, 0), 40)
+    with pytest.raises(mrad.InvalidArgument):
+        t.render_sample(np.zeros(48, np.float32), sc, cam, 33, 0, 1)
+    with pytest.raises(mrad.InvalidArgument):
+        t.render_sample(np.zeros(10, np.float32), sc, cam, 4, 0, 1)
+    with pytest.raises(mrad.InvalidArgument):
+        t.trace_replay(sc, np.zeros(5, np.float32))
+    rc = N.lib.mrad_upload_scene(None, None, 0)
+    assert rc == N.MRAD_ERR_INVALID_ARGUMENT
+    assert b"null" in N.lib.mrad_last_error()
+
+
+def test_destroy_null_is_noop():
+    N.lib.mrad_destroy(None)
+    N.lib.mrad_scene_destroy(None)
