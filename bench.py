@@ -1,17 +1,23 @@
-"""Benchmark of the MI355X path-tracing hot path.
+"""Benchmark of the MI355X path-tracing hot path (one GPU, one JSON line).
 
-Default workload: the heterogeneous 100k-primitive field (60 % spheres, 30 %
-boxes, 10 % affine), 1920x1080, 64 spp, 8 bounces, Russian roulette after
-bounce 3, rendered by the per-pixel megakernel on one GPU.  A step is one full
-frame (all 64 samples) with the scene resident in HBM and the frame left in
-HBM.  Prints one JSON line.
+Default workload (the configuration the headline metric is quoted on): the
+64-scene batch -- 64 seeded heterogeneous fields of 1k..1M primitives at
+256^2..1920x1080, 4/16/64 spp, 6 bounces, roulette after bounce 3 -- rendered
+by the batched wavefront form.  A step is one render of the whole batch with
+every scene resident in HBM and every image left in HBM.  Reported beside it:
 
-Roofline: the dominant kernel is the render kernel; its algorithmic bytes per
-frame come from the exact per-frame traversal counters of an instrumented run
-of the same frames (same seeds, bit-identical paths), priced per path segment
-as  24 N_aabb + 16 (N_internal + N_leaf) + 4 N_refs + 20 N_sphere + 28 N_box
-+ 88 (N_affine) + 104 N_plane + 52 N_segments + 12 bytes per pixel, divided by
-the kernel time measured with HIP events on the tracer's stream.
+  * the same batch rendered as 64 sequential plain-megakernel frames, and the
+    bit-identity of the two forms' images (so relMSE is equal by construction);
+  * the roofline of the dominant kernel (the wavefront trace kernel): exact
+    traversal counters of an instrumented run of the same batch priced per
+    path segment as 24 N_aabb + 16 (N_internal + N_leaf) + 4 N_refs + 20 N_sphere
+    + 28 N_box + 88 N_affine + 104 N_plane + 52 N_segments, divided by the
+    trace kernel's own time measured with HIP events on the batch stream;
+  * the adjoint/forward time ratio on the differentiable configuration (10k
+    primitives, 512^2, 32 spp, 4 bounces);
+  * the FP64 CPU renderer timed on a bounded sample on this host.
+
+--workload het runs the single 100k-primitive 1080p, 64 spp megakernel frame.
 """
 from __future__ import annotations
 
@@ -32,55 +38,175 @@ import mrad  # noqa: E402
 from mrad import configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Mpaths/s at fixed relMSE, wavefront batch vs megakernel; adjoint/fwd ratio"
 
 
-def algorithmic_bytes(c: dict, pixels: int) -> float:
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(c: dict) -> float:
     n_refs = c["primitive_tests"] - c["plane_tests"]
     return (24.0 * c["aabb_tests"] + 16.0 * (c["internal_node_visits"] + c["leaf_node_visits"]) + 4.0 * n_refs
             + 20.0 * c["sphere_tests"] + 28.0 * c["box_tests"]
             + 88.0 * (c["affine_sphere_tests"] + c["affine_box_tests"]) + 104.0 * c["plane_tests"]
-            + 52.0 * c["rays"] + 12.0 * pixels)
+            + 52.0 * c["rays"])
 
 
-def load_traffic(workload: str):
-    p = ROOT / "profiles" / f"traffic_{workload}.json"
+def load_traffic(name: str):
+    p = ROOT / "profiles" / f"traffic_{name}.json"
     if not p.exists():
         return None
     try:
-        d = json.loads(p.read_text())
-        return float(d["hbm_bytes_per_launch"])
+        return float(json.loads(p.read_text())["hbm_bytes_per_launch"])
     except Exception:
         return None
 
 
-def cpu_baseline(w: configs.Workload, seconds_hint: float = 15.0) -> dict:
-    """FP64 CPU restatement of the reference CPU renderer, all host threads (max 16)."""
+def cpu_baseline(w: configs.Workload, seconds: float = 12.0) -> dict:
+    """FP64 CPU restatement of the reference CPU renderer, all host threads (max 16), 1 spp frames."""
     from oracle import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     osc = O.Scene(w.scene.wire())
     cam = w.camera.packed()
-    # calibrate on a small band, then size the sample to ~seconds_hint
-    rows = 16
+    rows = max(8, w.height // 32)
     t0 = time.perf_counter()
     O.render64(osc, cam, w.width, rows, w.max_depth, 1, 1, rr_start=w.rr_start, threads=threads)
-    dt = max(1e-3, time.perf_counter() - t0)
-    rate = w.width * rows / dt
-    spp = 1
-    height = int(min(w.height, max(rows, rate * seconds_hint / w.width)))
+    rate = w.width * rows / max(1e-3, time.perf_counter() - t0)
+    spp = max(1, int(round(rate * seconds / (w.width * w.height))))
     t0 = time.perf_counter()
-    O.render64(osc, cam, w.width, height, w.max_depth, spp, 2, rr_start=w.rr_start, threads=threads)
+    O.render64(osc, cam, w.width, w.height, w.max_depth, spp, 2, rr_start=w.rr_start, threads=threads)
     dt = time.perf_counter() - t0
-    paths = w.width * height * spp
+    paths = w.width * w.height * spp
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"{w.width}x{height} band of the {w.name} frame, {spp} spp, FP64 CPU renderer, {dt:.1f} s"}
+            "sample": f"{w.name} {w.width}x{w.height} at {spp} spp, depth {w.max_depth}, FP64 CPU renderer, "
+                      f"{dt:.1f} s"}
 
 
-def bench_megakernel(args) -> dict:
+def make_batch(ws, **cfg) -> mrad.BatchRenderer:
+    b = mrad.BatchRenderer(mrad.WavefrontConfig(**cfg))
+    for w in ws:
+        b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, 0, w.frame_seed, w.rr_start)
+    return b
+
+
+def bench_batch(args) -> dict:
+    t0 = time.perf_counter()
+    ws = configs.batch(64, threads=16)
+    total_paths = sum(w.paths for w in ws)
+    prims = sum(w.scene.primitive_count() for w in ws)
+    log(f"built 64 scenes ({prims} primitives, {total_paths} paths) in {time.perf_counter() - t0:.1f} s")
+    cfg = dict(queue_capacity=args.queue_capacity, waves_per_cu=args.waves_per_cu)
+    b = make_batch(ws, **cfg)
+    for _ in range(args.warmup):
+        b.render()
+    wall, kern = [], []
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        st = b.render()
+        wall.append(time.perf_counter() - t1)
+        kern.append(st["kernel_nanos"] / 1e9)
+        log(f"wavefront step {i}: {wall[-1] * 1e3:.0f} ms, {st['steps']} bounce steps")
+    wf_images = [b.image(k) for k in range(len(ws))]
+    b.close()
+    value = total_paths * args.steps / sum(wall) / 1e6
+
+    # 64 sequential plain-megakernel renders of the same frames
+    tracers = []
+    for w in ws:
+        t = mrad.GpuRayTracer(w.width, w.height, w.spp, True)
+        t.set_russian_roulette(w.rr_start)
+        t.ensure_scene_resident(w.scene)
+        tracers.append(t)
+    mk_wall, mk_kern, identical = 0.0, 0.0, True
+    for k, (w, t) in enumerate(zip(ws, tracers)):
+        t.render_sample(None, w.scene, w.camera, w.max_depth, 0, w.frame_seed)   # warm
+    for k, (w, t) in enumerate(zip(ws, tracers)):
+        t1 = time.perf_counter()
+        t.render_sample(None, w.scene, w.camera, w.max_depth, 0, w.frame_seed)
+        mk_wall += time.perf_counter() - t1
+        mk_kern += t.last_frame_stats()["kernel_nanos"] / 1e9
+        img = np.zeros(w.width * w.height * 3, np.float32)
+        t.copy_frame(img)
+        identical &= bool(np.array_equal(img.view(np.uint32), wf_images[k].view(np.uint32)))
+    for t in tracers:
+        t.close()
+    mk_value = total_paths / mk_wall / 1e6
+    log(f"megakernel: {mk_wall * 1e3:.0f} ms for 64 renders, images bit-identical: {identical}")
+
+    # dominant kernel: per-kernel event timing and exact counters of the same batch
+    bt = make_batch(ws, time_kernels=True, **cfg)
+    st_t = bt.render()
+    bt.close()
+    bm = make_batch(ws, collect_metrics=True, **cfg)
+    st_m = bm.render()
+    bm.close()
+    trace_s = st_t["trace_nanos"] / 1e9
+    achieved = algorithmic_bytes(st_m) / trace_s / 1e9
+    log(f"trace kernel {trace_s * 1e3:.0f} ms of {st_t['kernel_nanos'] / 1e6:.0f} ms; achieved {achieved:.0f} GB/s")
+
+    adj = bench_adjoint()
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sum(wall) / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "batch-64 wavefront", "scenes": 64, "primitives": prims, "paths_per_step": total_paths,
+                   "resolutions": "256^2..1920x1080", "spp": "4/16/64", "max_depth": 6, "russian_roulette_after": 3,
+                   "queue_capacity": args.queue_capacity, "waves_per_cu": args.waves_per_cu, "hipgraph": True},
+        "megakernel_mpaths_per_s": mk_value,
+        "wavefront_over_megakernel": value / mk_value,
+        "kernel_only": {"wavefront_ms": float(np.mean(kern)) * 1e3, "megakernel_ms": mk_kern * 1e3,
+                        "ratio": mk_kern / float(np.mean(kern))},
+        "images_bit_identical": identical,
+        "relmse_equal": identical,
+        "mrays_per_s": st_m["rays"] / (sum(wall) / args.steps) / 1e6,
+        "segments_per_path": st_m["rays"] / total_paths,
+        "adjoint": adj,
+        "adjoint_over_forward": adj["ratio"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("batch-64"),
+                     "kernel": "wfTrace", "kernel_ms_per_step": trace_s * 1e3},
+        "counters_per_step": {k: st_m[k] for k in mrad.STAT_NAMES},
+    }, ws
+
+
+def bench_adjoint() -> dict:
+    """Differentiable configuration: forward and path-replay backward kernel times."""
+    from mrad.adjoint import PathReplayRenderer, l2_loss
+    w = configs.diff(10_000)
+    r = PathReplayRenderer(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, frame_seed=0xD1FF)
+    rng = np.random.default_rng(9)
+    m = r.base_materials.copy()
+    m[:, 0:3] *= rng.uniform(0.8, 1.2, (len(m), 3)).astype(np.float32)
+    m[:, 9:12] *= rng.uniform(0.8, 1.2, (len(m), 1)).astype(np.float32)
+    target = r.forward(m)
+    img = r.forward(r.base_materials)
+    _, g = l2_loss(img, target)
+    r.backward(g)
+    fwd, bwd = [], []
+    for _ in range(3):
+        r.forward()
+        fwd.append(r.last_kernel_nanos())
+        r.backward(g)
+        bwd.append(r.last_kernel_nanos())
+    r.close()
+    f, bk = float(np.median(fwd)) / 1e6, float(np.median(bwd)) / 1e6
+    return {"workload": "diff-10k 512x512 32 spp 4 bounces", "forward_ms": f, "backward_ms": bk, "ratio": bk / f}
+
+
+def bench_het(args) -> dict:
     w = configs.het(100_000)
     stats = w.scene.bvh_stats()
     t = mrad.GpuRayTracer(w.width, w.height, w.spp, True)
-    if not t.is_available():
-        raise SystemExit("no HIP device available: " + mrad._native.lib.mrad_last_error().decode())
     t.set_russian_roulette(w.rr_start)
     cam = w.camera.packed()
     seed = 0x4D524144
@@ -94,7 +220,6 @@ def bench_megakernel(args) -> dict:
         kernel_ns.append(t.last_frame_stats()["kernel_nanos"])
     wall = time.perf_counter() - t0
     t.close()
-    # instrumented replay of the same frames for the exact per-frame counters
     tm = mrad.GpuRayTracer(w.width, w.height, w.spp, True, collect_metrics=True)
     tm.set_russian_roulette(w.rr_start)
     totals = None
@@ -102,56 +227,47 @@ def bench_megakernel(args) -> dict:
         fi = args.warmup + i
         tm.render_sample(None, w.scene, cam, w.max_depth, fi, mrad.frame_seed(seed, fi))
         st = tm.last_frame_stats()
-        totals = st if totals is None else {k: totals[k] + st[k] if k != "max_stack_size" else max(totals[k], st[k])
-                                            for k in st}
+        totals = st if totals is None else {k: (totals[k] + st[k]) if k != "max_stack_size"
+                                            else max(totals[k], st[k]) for k in st}
     tm.close()
     paths = w.paths * args.steps
-    segs = totals["rays"]
     kernel_s = sum(kernel_ns) / 1e9
-    achieved = algorithmic_bytes(totals, w.width * w.height * args.steps) / kernel_s / 1e9
-    traffic = load_traffic("het-100k")
-    res = {
-        "metric": "Mpaths/s at fixed relMSE, wavefront batch vs megakernel; adjoint/fwd ratio",
-        "value": paths / wall / 1e6,
-        "unit": "Mpaths/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded scene generator)",
+    achieved = (algorithmic_bytes(totals) + 12.0 * w.width * w.height * args.steps) / kernel_s / 1e9
+    return {
+        "metric": METRIC, "value": paths / wall / 1e6, "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": {"workload": "het-100k megakernel", "primitives": 100_000, "width": w.width, "height": w.height,
                    "spp": w.spp, "max_depth": w.max_depth, "russian_roulette_after": w.rr_start,
-                   "bvh": f"weighted SAH leaf 8, {stats.node_count} nodes", "form": "megakernel"},
-        "mrays_per_s": segs / wall / 1e6,
-        "segments_per_path": segs / paths,
-        "kernel_ms_per_step": kernel_s / args.steps * 1e3,
+                   "bvh": f"weighted SAH leaf 8, {stats.node_count} nodes"},
+        "mrays_per_s": totals["rays"] / wall / 1e6, "segments_per_path": totals["rays"] / paths,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("het-100k"), "kernel": "renderKernel",
+                     "kernel_ms_per_step": kernel_s / args.steps * 1e3},
         "counters_per_step": {k: totals[k] / args.steps for k in mrad.STAT_NAMES},
-    }
-    return res
+    }, [w]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["batch", "het"], default="batch")
+    ap.add_argument("--queue-capacity", type=int, default=1 << 23)
+    ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: the path renders each frame on one GPU and this build runs no "
+        print("bench.py measures one MI355X: every frame and batch renders on one GPU and this build has no "
               "multi-GPU form; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = bench_megakernel(args)
-    if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(configs.het(100_000))
-    else:
+    res, ws = bench_batch(args) if args.workload == "batch" else bench_het(args)
+    if args.no_cpu_baseline:
         res["cpu_baseline"] = None
+    else:
+        sample = configs.het(100_000) if args.workload == "het" else ws[max(range(len(ws)), key=lambda i: ws[i].scene.primitive_count() if ws[i].width == 1920 else 0)]
+        res["cpu_baseline"] = cpu_baseline(sample)
     print(json.dumps(res), flush=True)
 
 

@@ -185,6 +185,70 @@ int mrad_validate_full_path_depth(int max_depth);
 /* Message of the last failed call on this thread ("" after success). */
 const char *mrad_last_error(void);
 
+/* ------------------------------------------------------ path-replay adjoint */
+/* No counterpart in the reference.  Backward pass of mrad_render_sample for
+ * the resident scene and this tracer's width, height, samples and roulette:
+ * given grad_image = dLoss/dImage (W*H*3 floats, host), returns dLoss/d(diffuse
+ * albedo) and dLoss/d(emittance) per primitive (primitive_count x 3 floats each,
+ * primitives in the reference's global order: spheres, boxes, affine spheres,
+ * affine boxes, planes).  Paths are re-traced from their seeds, not stored. */
+int mrad_backward(mrad_tracer *tracer, const float *grad_image, const float camera[12], int max_depth,
+                  int frame_index, uint64_t frame_seed, float *grad_albedo, float *grad_emission,
+                  int primitive_count);
+/* Replaces the resident material table (13 floats each, same count) until
+ * another scene or revision is uploaded: the parameter update of an
+ * optimisation step.  No counterpart in the reference. */
+int mrad_update_materials(mrad_tracer *tracer, const float *materials13, int count);
+
+/* ------------------------------------------------------ wavefront batch */
+/* No counterpart in the reference (its only form is the per-pixel
+ * renderKernel): many scenes rendered by one bounded pool of in-flight paths,
+ * one persistent kernel per bounce step, captured as a hipGraph.  Every image
+ * is bit-identical to mrad_render_sample's for the same view. */
+
+/* One scene's render request: the renderSample arguments plus the
+ * GpuRayTracer constructor's width, height and samples per frame. */
+typedef struct {
+	int width, height, samples;
+	int max_depth;           /* 0..32 */
+	int rr_start;            /* Russian roulette start depth, -1 = none */
+	int frame_index;
+	uint64_t frame_seed;
+	float camera[12];        /* position, forward, right, up */
+} mrad_view;
+
+typedef struct {
+	int device;
+	int queue_capacity;      /* paths in flight per pool (0 = 4M) */
+	int waves_per_cu;        /* persistent-grid occupancy (0 = 16) */
+	int bvh_stack_size;      /* 8..256 (0 = 32) */
+	int linear_scan;         /* 1 = the reference's GPU_LINEAR scan instead of the BVH */
+	int collect_metrics;     /* counter variant of the trace kernel */
+	int use_graph;           /* capture the bounce steps as a hipGraph */
+	int time_kernels;        /* time each kernel with HIP events (no graph; for measurement) */
+	int lds_stack_entries;   /* traversal-stack entries per lane held in LDS: 4 or 8 (0 = 8) */
+} mrad_wavefront_config;
+
+typedef struct {
+	int64_t total_nanos, kernel_nanos, upload_nanos;
+	int64_t steps;           /* bounce steps run (generate + trace + shade) */
+	uint64_t paths;
+	uint64_t counters[MRAD_STAT_COUNT];
+	int64_t trace_nanos, shade_nanos, generate_nanos;   /* with time_kernels */
+} mrad_batch_stats;
+
+typedef struct mrad_batch mrad_batch;
+
+int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out);
+void mrad_batch_destroy(mrad_batch *batch);
+/* Appends a scene (wire arrays copied) with its view; index_out may be NULL. */
+int mrad_batch_add_scene(mrad_batch *batch, const mrad_scene_arrays *scene, const mrad_view *view, int *index_out);
+/* Renders every scene of the batch; images stay on the device. */
+int mrad_batch_render(mrad_batch *batch);
+/* Normalized RGB image (width*height*3 floats) of one scene. */
+int mrad_batch_copy_image(mrad_batch *batch, int scene, float *rgb_out);
+int mrad_batch_get_stats(mrad_batch *batch, mrad_batch_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

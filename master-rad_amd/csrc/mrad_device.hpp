@@ -14,6 +14,12 @@
 //     closest hit are recomputed once, after traversal, by the same
 //     arithmetic on the winning primitive, so the per-test fetch of an affine
 //     primitive is 48 bytes (inverse rows with the centre) instead of 88.
+//   * All scenes of a batch share one arena: node, ref and primitive arrays
+//     are concatenated and their indices rebased on upload; a SceneRange
+//     names one scene's slices.  Within a scene, primitive order is compared
+//     through a rank (type << 30 | arena index), which orders exactly like the
+//     reference's global order (spheres, boxes, affine spheres, affine
+//     boxes); the reference's order number is rebuilt from it on output.
 //
 // Accept rules are the reference's: BVH leaves test against
 // nextafterf(best.t, +inf) and accept on (t, order) (GpuKernelSources.java:772-806);
@@ -114,13 +120,26 @@ __device__ __forceinline__ uint32_t path_seed(uint64_t frameSeed, int frameIndex
 	return hash_u32(seed);
 }
 
-// ------------------------------------------------------------ device scene
+// Camera ray of renderKernel (GpuKernelSources.java:1256-1264).
+__device__ __forceinline__ void cameraRay(const float *cam, int width, int height, int x, int y, uint32_t &rng, f3 &o,
+                                          f3 &d) {
+	const float jx = rand01(rng) - 0.5f;
+	const float jy = rand01(rng) - 0.5f;
+	const float aspect = (float) width / (float) height;
+	const float u = ((((float) x + 0.5f) + jx) * (2.0f / (float) width) - 1.0f) * aspect;
+	const float v = 1.0f - ((((float) y + 0.5f) + jy) * (2.0f / (float) height));
+	o = mk(cam[0], cam[1], cam[2]);
+	d = mk(cam[3], cam[4], cam[5]) + mk(cam[6], cam[7], cam[8]) * u + mk(cam[9], cam[10], cam[11]) * v;
+}
+
+// ------------------------------------------------------------ device arena
 
 struct MaterialF { f3 diffuse, reflective, refractive, emittance; float ior; };
 
+// Global arrays of the arena (one scene or a whole batch).
 struct DevScene {
 	const float4 *nodes;        // 4 x float4 per node (see header comment)
-	const uint32_t *refs;
+	const uint32_t *refs;       // 2-bit type | 30-bit arena index
 	const float4 *spheres;      // {centre, radius}
 	const int *sphereMat;
 	const float4 *boxes;        // 2 per box: {lo.xyz, hi.x}, {hi.y, hi.z, mat bits, 0}
@@ -130,12 +149,19 @@ struct DevScene {
 	const float4 *affBN;
 	const float *planes;        // wire format, 26 floats per plane
 	const float4 *materials;    // 4 float4 per material: diffuse, reflective, refractive, emittance+ior
-	int sphereCount, boxCount, affSCount, affBCount, planeCount;
-	int nodeCount, refCount, rootCount, useBvh, materialCount;
+};
+
+// One scene's slices of the arena and its traversal parameters.
+struct SceneRange {
+	int rootCount;
 	int roots[4];
 	float rootBounds[4][6];
-	float bg[3];
+	int refBase, refCount;
+	int sphBase, sphCount, boxBase, boxCount, asBase, asCount, abBase, abCount;
+	int planeBase, planeCount;
+	int useBvh;                 // BVH traversal (1) or linear scan (0)
 	int stackLimit;             // BVH_STACK_SIZE semantics
+	float bg[3];
 };
 
 __device__ __forceinline__ MaterialF loadMaterial(const DevScene &S, int index) {
@@ -355,7 +381,9 @@ __device__ __forceinline__ float entryDistance(float lx, float ly, float lz, flo
 
 // ------------------------------------------------------------ closest hit
 
-struct Best { float t; int order; int kind; int index; };
+// Best candidate: distance, rank (type << 30 | arena index; planes use kind 4
+// and never enter a rank comparison), kind and arena index.
+struct Best { float t; uint32_t rank; int kind; int index; };
 
 template <bool kMetrics>
 struct Counters {
@@ -385,22 +413,23 @@ __device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_
 	const float limit = nextafterf(best.t, INFINITY);
 	float t = 0.0f;
 	bool found;
-	int order, kind;
+	int kind;
 	if (type == kRefBox) {
-		order = S.sphereCount + index; kind = kKindBox;
+		kind = kKindBox;
 		found = testBox(S.boxes[2 * index], S.boxes[2 * index + 1], o, d, limit, t);
 	} else if (type == kRefAffineSphere) {
-		order = S.sphereCount + S.boxCount + index; kind = kKindAffineSphere;
+		kind = kKindAffineSphere;
 		found = testAffineSphere(S.affS + 3 * index, o, d, limit, t);
 	} else if (type == kRefAffineBox) {
-		order = S.sphereCount + S.boxCount + S.affSCount + index; kind = kKindAffineBox;
+		kind = kKindAffineBox;
 		found = testAffineBox(S.affB + 3 * index, o, d, limit, t);
 	} else {
-		order = index; kind = kKindSphere;
+		kind = kKindSphere;
 		found = testSphere(S.spheres[index], o, d, limit, t);
 	}
-	if (!found || t > best.t || (t == best.t && order >= best.order)) return;
-	best.t = t; best.order = order; best.kind = kind; best.index = index;
+	const uint32_t rank = type | (uint32_t) index;
+	if (!found || t > best.t || (t == best.t && rank >= best.rank)) return;
+	best.t = t; best.rank = rank; best.kind = kind; best.index = index;
 }
 
 // Traversal stack storage.  The top kShortStack entries of each lane live in
@@ -418,40 +447,43 @@ struct LdsStack {
 	float *entry;
 };
 
-template <int kCap>
+// kShort is the number of top entries held in LDS (the LDS split); kCap the
+// total storage capacity.
+template <int kCap, int kShort = kShortStack>
 struct TravStack {
-	static constexpr int kSpill = kCap > kShortStack ? kCap - kShortStack : 1;
+	static constexpr int kSpill = kCap > kShort ? kCap - kShort : 1;
 	LdsStack lds;
 	int spillNode[kSpill];
 	float spillEntry[kSpill];
 	__device__ __forceinline__ void set(int i, int n, float e) {
-		if (kShortStack > 0 && i < kShortStack) { lds.node[i * 64] = n; lds.entry[i * 64] = e; }
-		else { spillNode[i - kShortStack] = n; spillEntry[i - kShortStack] = e; }
+		if (kShort > 0 && i < kShort) { lds.node[i * 64] = n; lds.entry[i * 64] = e; }
+		else { spillNode[i - kShort] = n; spillEntry[i - kShort] = e; }
 	}
 	__device__ __forceinline__ int node(int i) const {
-		return (kShortStack > 0 && i < kShortStack) ? lds.node[i * 64] : spillNode[i - kShortStack];
+		return (kShort > 0 && i < kShort) ? lds.node[i * 64] : spillNode[i - kShort];
 	}
 	__device__ __forceinline__ float entry(int i) const {
-		return (kShortStack > 0 && i < kShortStack) ? lds.entry[i * 64] : spillEntry[i - kShortStack];
+		return (kShort > 0 && i < kShort) ? lds.entry[i * 64] : spillEntry[i - kShort];
 	}
 };
 
 // traceBvh (GpuKernelSources.java:844-961): same stack discipline, same
-// counters.  kCap is the storage capacity; the runtime limit S.stackLimit
+// counters.  kCap is the storage capacity; the runtime limit R.stackLimit
 // (<= kCap) gives BVH_STACK_SIZE semantics.
-template <bool kMetrics, int kCap>
-__device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c, LdsStack lds) {
-	if (S.nodeCount <= 0 || S.refCount <= 0 || S.rootCount <= 0 || S.rootCount > 4) return;
-	TravStack<kCap> st;
+template <bool kMetrics, int kCap, int kShort = kShortStack>
+__device__ inline void traceBvh(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
+                                Counters<kMetrics> &c, LdsStack lds) {
+	if (R.refCount <= 0 || R.rootCount <= 0 || R.rootCount > 4) return;
+	TravStack<kCap, kShort> st;
 	st.lds = lds;
-	const int cap = S.stackLimit;
+	const int cap = R.stackLimit;
 	int size = 0;
 	bool overflow = false;
-	for (int r = 0; r < S.rootCount; r++) {
-		int ri = S.roots[r];
-		if (ri < 0 || ri >= S.nodeCount) { overflow = true; break; }
+	for (int r = 0; r < R.rootCount; r++) {
+		const int ri = R.roots[r];
+		if (ri < 0) { overflow = true; break; }
 		c.add(1, 1);
-		const float *rb = S.rootBounds[r];
+		const float *rb = R.rootBounds[r];
 		float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, d, best.t);
 		if (!isfinite(e)) continue;
 		if (size >= cap) { overflow = true; break; }
@@ -482,7 +514,7 @@ __device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Count
 			                        : kind == -4 ? kRefAffineSphere : kRefAffineBox;
 			for (int i = 0; i < nd.w; i++) {
 				const int ri = nd.z + i;
-				if (ri < 0 || ri >= S.refCount) continue;
+				if (ri < R.refBase || ri >= R.refBase + R.refCount) continue;
 				const uint32_t ref = S.refs[ri];
 				const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
 				c.add(3 + typeSlot(type), 1);
@@ -513,7 +545,7 @@ __device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Count
 	}
 	if (overflow) {
 		c.add(8, 1);
-		for (int i = 0; i < S.refCount; i++) {
+		for (int i = R.refBase; i < R.refBase + R.refCount; i++) {
 			const uint32_t ref = S.refs[i];
 			c.add(2, 1);
 			c.add(3 + typeSlot(ref & kRefTypeMask), 1);
@@ -525,35 +557,46 @@ __device__ inline void traceBvh(const DevScene &S, f3 o, f3 d, Best &best, Count
 // Linear scan of every primitive class in global order, strict against the
 // running best (GpuKernelSources.java:993-1013).
 template <bool kMetrics>
-__device__ inline void traceLinear(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c) {
-	c.add(2, (uint32_t) (S.sphereCount + S.boxCount + S.affSCount + S.affBCount));
-	c.add(3, (uint32_t) S.sphereCount);
-	c.add(4, (uint32_t) S.boxCount);
-	c.add(6, (uint32_t) S.affSCount);
-	c.add(7, (uint32_t) S.affBCount);
+__device__ inline void traceLinear(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
+                                   Counters<kMetrics> &c) {
+	c.add(2, (uint32_t) (R.sphCount + R.boxCount + R.asCount + R.abCount));
+	c.add(3, (uint32_t) R.sphCount);
+	c.add(4, (uint32_t) R.boxCount);
+	c.add(6, (uint32_t) R.asCount);
+	c.add(7, (uint32_t) R.abCount);
 	float t;
-	for (int i = 0; i < S.sphereCount; i++)
-		if (testSphere(S.spheres[i], o, d, best.t, t)) best = {t, i, kKindSphere, i};
-	for (int i = 0; i < S.boxCount; i++)
-		if (testBox(S.boxes[2 * i], S.boxes[2 * i + 1], o, d, best.t, t)) best = {t, S.sphereCount + i, kKindBox, i};
-	for (int i = 0; i < S.affSCount; i++)
-		if (testAffineSphere(S.affS + 3 * i, o, d, best.t, t)) best = {t, S.sphereCount + S.boxCount + i, kKindAffineSphere, i};
-	for (int i = 0; i < S.affBCount; i++)
-		if (testAffineBox(S.affB + 3 * i, o, d, best.t, t))
-			best = {t, S.sphereCount + S.boxCount + S.affSCount + i, kKindAffineBox, i};
+	for (int i = R.sphBase; i < R.sphBase + R.sphCount; i++)
+		if (testSphere(S.spheres[i], o, d, best.t, t)) best = {t, kRefSphere | (uint32_t) i, kKindSphere, i};
+	for (int i = R.boxBase; i < R.boxBase + R.boxCount; i++)
+		if (testBox(S.boxes[2 * i], S.boxes[2 * i + 1], o, d, best.t, t)) best = {t, kRefBox | (uint32_t) i, kKindBox, i};
+	for (int i = R.asBase; i < R.asBase + R.asCount; i++)
+		if (testAffineSphere(S.affS + 3 * i, o, d, best.t, t))
+			best = {t, kRefAffineSphere | (uint32_t) i, kKindAffineSphere, i};
+	for (int i = R.abBase; i < R.abBase + R.abCount; i++)
+		if (testAffineBox(S.affB + 3 * i, o, d, best.t, t)) best = {t, kRefAffineBox | (uint32_t) i, kKindAffineBox, i};
 }
 
 // tracePlane (GpuKernelSources.java:708-734) distance part.
-__device__ inline void tracePlanes(const DevScene &S, f3 o, f3 d, Best &best) {
-	const int base = S.sphereCount + S.boxCount + S.affSCount + S.affBCount;
-	for (int i = 0; i < S.planeCount; i++) {
+__device__ inline void tracePlanes(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best) {
+	for (int i = R.planeBase; i < R.planeBase + R.planeCount; i++) {
 		const float *p = S.planes + 26 * i;
 		f3 n = mk(p[0], p[1], p[2]);
 		float denom = dot(n, d);
 		if (denom == 0.0f) continue;
 		float t = -(dot(n, o) + p[3]) / denom;
 		if (t <= kEps || t >= best.t) continue;
-		best = {t, base + i, kKindPlane, i};
+		best = {t, 0xFFFFFFFFu, kKindPlane, i};
+	}
+}
+
+// The reference's global primitive order of the best hit (GpuKernelSources.java:784-799, :1017).
+__device__ __forceinline__ int localOrder(const SceneRange &R, const Best &b) {
+	switch (b.kind) {
+		case kKindSphere: return b.index - R.sphBase;
+		case kKindBox: return R.sphCount + b.index - R.boxBase;
+		case kKindAffineSphere: return R.sphCount + R.boxCount + b.index - R.asBase;
+		case kKindAffineBox: return R.sphCount + R.boxCount + R.asCount + b.index - R.abBase;
+		default: return R.sphCount + R.boxCount + R.asCount + R.abCount + b.index - R.planeBase;
 	}
 }
 
@@ -577,25 +620,25 @@ __device__ inline int planeMaterialIndex(const float *p, f3 hp) {
 struct HitDetail { f3 pos, normal; int matIndex; };
 
 // Normal, position and material of the winning primitive, recomputed with
-// the arithmetic of the routine that accepted it.
-__device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, const Best &best) {
+// the arithmetic of the routine that accepted it.  Material indices in the
+// arena are already rebased to the arena's material table.
+__device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, float t, int kind, int i) {
 	HitDetail h;
-	h.pos = o + d * best.t;
-	const int i = best.index;
-	if (best.kind == kKindSphere) {
+	h.pos = o + d * t;
+	if (kind == kKindSphere) {
 		SphereSolve r;
 		sphereSolve(S.spheres[i], o, d, r);
 		h.normal = normalizeSafe(mk(fmaf(d.x, r.radialOffset, r.closest.x), fmaf(d.y, r.radialOffset, r.closest.y),
 		                            fmaf(d.z, r.radialOffset, r.closest.z)));
 		h.matIndex = S.sphereMat[i];
-	} else if (best.kind == kKindBox) {
+	} else if (kind == kKindBox) {
 		float4 b1 = S.boxes[2 * i + 1];
 		BoxSolve r;
 		boxSolve(S.boxes[2 * i], b1, o, d, r);
 		h.normal = axisNormal(r.axis, r.sign);
 		h.matIndex = __float_as_int(b1.z);
-	} else if (best.kind == kKindAffineSphere || best.kind == kKindAffineBox) {
-		const bool isBox = best.kind == kKindAffineBox;
+	} else if (kind == kKindAffineSphere || kind == kKindAffineBox) {
+		const bool isBox = kind == kKindAffineBox;
 		const float4 *rec = (isBox ? S.affB : S.affS) + 3 * i;
 		const float4 *nrm = (isBox ? S.affBN : S.affSN) + 3 * i;
 		AffineLocal a;
@@ -626,18 +669,18 @@ __device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, const Best
 }
 
 // traceClosest (GpuKernelSources.java:963-1020).
-template <bool kMetrics, int kCap>
-__device__ __forceinline__ bool traceClosest(const DevScene &S, f3 o, f3 d, Best &best, Counters<kMetrics> &c,
-                                             LdsStack lds) {
+template <bool kMetrics, int kCap, int kShort = kShortStack>
+__device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
+                                             Counters<kMetrics> &c, LdsStack lds) {
 	best.t = 1e30f;
-	best.order = 0x7FFFFFFF;
+	best.rank = 0xFFFFFFFFu;
 	best.kind = -1;
 	best.index = -1;
-	if (S.useBvh != 0 && S.nodeCount > 0) traceBvh<kMetrics, kCap>(S, o, d, best, c, lds);
-	else traceLinear<kMetrics>(S, o, d, best, c);
-	c.add(2, (uint32_t) S.planeCount);
-	c.add(5, (uint32_t) S.planeCount);
-	tracePlanes(S, o, d, best);
+	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort>(S, R, o, d, best, c, lds);
+	else traceLinear<kMetrics>(S, R, o, d, best, c);
+	c.add(2, (uint32_t) R.planeCount);
+	c.add(5, (uint32_t) R.planeCount);
+	tracePlanes(S, R, o, d, best);
 	return best.kind >= 0;
 }
 
@@ -696,27 +739,35 @@ __device__ __forceinline__ bool roulette(int depth, int maxDepth, int rrStart, f
 	return true;
 }
 
+// One path segment after its closest-hit query: emission, scattering,
+// throughput and roulette.  Returns false when the path ends here.
+__device__ __forceinline__ bool shadeSegment(const DevScene &S, const SceneRange &R, bool found, const Best &best,
+                                             int depth, int maxDepth, int rrStart, f3 &o, f3 &d, f3 &T, f3 &L,
+                                             uint32_t &rng) {
+	if (!found) {
+		L = L + T * mk(R.bg[0], R.bg[1], R.bg[2]);
+		return false;
+	}
+	HitDetail h = resolveHit(S, o, d, best.t, best.kind, best.index);
+	MaterialF m = loadMaterial(S, h.matIndex);
+	L = L + T * m.emittance;
+	f3 w;
+	if (!scatter(m, h.normal, d, rng, w)) return false;
+	o = h.pos + d * kEps;
+	T = T * w;
+	return roulette(depth, maxDepth, rrStart, T, rng);
+}
+
 // tracePath (GpuKernelSources.java:1061-1165).
 template <bool kMetrics, int kCap>
-__device__ inline f3 tracePath(const DevScene &S, f3 o, f3 d, int maxDepth, int rrStart, uint32_t &rng,
-                               Counters<kMetrics> &c, LdsStack lds) {
+__device__ inline f3 tracePath(const DevScene &S, const SceneRange &R, f3 o, f3 d, int maxDepth, int rrStart,
+                               uint32_t &rng, Counters<kMetrics> &c, LdsStack lds) {
 	f3 T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
-	const f3 bg = mk(S.bg[0], S.bg[1], S.bg[2]);
 	for (int depth = 0; depth < maxDepth; ++depth) {
 		c.add(0, 1);
 		Best best;
-		if (!traceClosest<kMetrics, kCap>(S, o, d, best, c, lds)) {
-			L = L + T * bg;
-			break;
-		}
-		HitDetail h = resolveHit(S, o, d, best);
-		MaterialF m = loadMaterial(S, h.matIndex);
-		L = L + T * m.emittance;
-		f3 w;
-		if (!scatter(m, h.normal, d, rng, w)) break;
-		o = h.pos + d * kEps;
-		T = T * w;
-		if (!roulette(depth, maxDepth, rrStart, T, rng)) break;
+		const bool found = traceClosest<kMetrics, kCap>(S, R, o, d, best, c, lds);
+		if (!shadeSegment(S, R, found, best, depth, maxDepth, rrStart, o, d, T, L, rng)) break;
 	}
 	return L;
 }

@@ -1,0 +1,42 @@
+// The tracer handle behind mrad_create/mrad_destroy, shared by the render,
+// replay and adjoint translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mrad_host.hpp"
+
+struct mrad_tracer {
+	std::mutex mu;
+	mrad_tracer_config cfg{};
+	bool available = false;
+	std::string unavailableReason;
+	hipStream_t stream = nullptr;
+	hipEvent_t evStart = nullptr, evEnd = nullptr;
+	std::vector<hipEvent_t> boundary;
+	mrad::DevArena arena;
+	mrad::SceneRange range{};
+	int materialCount = 0;
+	bool hasScene = false;
+	uint64_t residentKey = 0, residentRevision = 0;
+	mrad::DevBuf output, stats, rays, flags, dist, order, mat, normals, perRay;
+	mrad::DevBuf gradImage, gradAlbedo, gradEmission;
+	int rrStart = -1;
+	mrad_frame_stats last{};
+};
+
+namespace mrad {
+
+inline int requireTracer(mrad_tracer *t) {
+	if (!t) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle is null");
+	if (!t->available)
+		return set_error(MRAD_ERR_UNAVAILABLE, "HIP path tracer is unavailable: " + t->unavailableReason);
+	return MRAD_OK;
+}
+
+inline int primitiveCount(const SceneRange &R) { return R.sphCount + R.boxCount + R.asCount + R.abCount + R.planeCount; }
+
+}  // namespace mrad

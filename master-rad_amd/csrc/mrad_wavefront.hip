@@ -1,0 +1,588 @@
+// Batched wavefront form of the path tracer (no counterpart in the
+// reference, whose renderer is a per-pixel megakernel,
+// gpu/GpuKernelSources.java:1167-1315).
+//
+// Many scenes of uneven size and resolution share one arena and one bounded
+// pool of in-flight paths.  Each bounce step is three persistent kernels:
+//   generate  fills the pool up to its capacity with new camera paths, taken
+//             in global path order across all scenes;
+//   trace     closest hit of every live path (the megakernel's traversal);
+//   shade     emission, scattering and roulette of every live path; the
+//             survivors are compacted into the other pool by a wave64 ballot
+//             and prefix sum with one atomic per wave, finished paths write
+//             their radiance to a per-path slot.
+// Two bounce steps (pool A to B and back) are captured once as a hipGraph and
+// replayed until every path has finished; a resolve kernel then sums each
+// pixel's samples in sample order.  Every path runs exactly the megakernel's
+// arithmetic and every pixel sums its samples in the megakernel's order, so
+// both forms produce bit-identical images.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mrad_host.hpp"
+#include "mrad_kernels_common.hpp"
+
+namespace mrad {
+
+struct WfScene {
+	SceneRange R;
+	float cam[12];
+	int width, height, spp, maxDepth, rrStart, frameIndex;
+	uint64_t frameSeed;
+	uint32_t pathBase;      // first global path id of the scene
+	uint32_t pixelBase;     // first global pixel of the scene
+};
+
+// Path pool, structure of arrays of 16-byte records:
+//   od0 = (o.x, o.y, o.z, d.x)   od1 = (d.y, d.z, T.x, T.y)
+//   tl  = (T.z, L.x, L.y, L.z)   meta = (rng, path id, scene | depth << 16, 0)
+struct WfPool {
+	float4 *od0, *od1, *tl;
+	uint4 *meta;
+};
+
+// Device counter block.
+enum { kCount0 = 0, kCount1 = 1, kNextPath = 2, kWorkTrace = 3, kWorkShade = 4, kCounterWords = 16 };
+constexpr uint32_t kDead = 0xFFFFu;          // depth field of a path that never traces
+constexpr uint32_t kMissKind = 7u;
+
+struct WfParams {
+	DevScene S;
+	const WfScene *scenes;
+	int sceneCount;
+	WfPool pool[2];
+	float *hitT;
+	uint32_t *hitKI;        // kind << 29 | arena index
+	float *results;         // 3 floats per path
+	float *images;          // 3 floats per pixel, all scenes
+	uint32_t *counters;
+	uint32_t totalPaths;
+	uint32_t totalPixels;
+	uint32_t capacity;
+	unsigned long long *stats;
+};
+
+__device__ __forceinline__ int sceneOfPath(const WfParams &P, uint32_t g) {
+	int lo = 0, hi = P.sceneCount - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (P.scenes[mid].pathBase <= g) lo = mid; else hi = mid - 1;
+	}
+	return lo;
+}
+__device__ __forceinline__ int sceneOfPixel(const WfParams &P, uint32_t p) {
+	int lo = 0, hi = P.sceneCount - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (P.scenes[mid].pixelBase <= p) lo = mid; else hi = mid - 1;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ uint32_t genCount(const WfParams &P, int cur) {
+	const uint32_t base = P.counters[kCount0 + cur];
+	const uint32_t next = P.counters[kNextPath];
+	const uint32_t free = P.capacity > base ? P.capacity - base : 0u;
+	const uint32_t left = P.totalPaths - next;
+	return free < left ? free : left;
+}
+
+// Camera paths [next, next + n) appended to pool `cur` (renderKernel's seed
+// and camera ray, GpuKernelSources.java:1231-1264).
+__global__ __launch_bounds__(kBlock) void wfGenerate(WfParams P, int cur) {
+	const uint32_t base = P.counters[kCount0 + cur];
+	const uint32_t next = P.counters[kNextPath];
+	const uint32_t n = genCount(P, cur);
+	const WfPool Q = P.pool[cur];
+	for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+		const uint32_t g = next + j;
+		const int k = sceneOfPath(P, g);
+		const WfScene &V = P.scenes[k];
+		const uint32_t local = g - V.pathBase;
+		const uint32_t pixel = local / (uint32_t) V.spp, s = local % (uint32_t) V.spp;
+		const int x = (int) (pixel % (uint32_t) V.width), y = (int) (pixel / (uint32_t) V.width);
+		uint32_t rng = path_seed(V.frameSeed, V.frameIndex, pixel, (int) s);
+		f3 o, d;
+		cameraRay(V.cam, V.width, V.height, x, y, rng, o, d);
+		uint32_t depth = 0;
+		if (V.maxDepth == 0) {
+			depth = kDead;
+			P.results[3 * (size_t) g] = 0.0f;
+			P.results[3 * (size_t) g + 1] = 0.0f;
+			P.results[3 * (size_t) g + 2] = 0.0f;
+		}
+		const uint32_t slot = base + j;
+		Q.od0[slot] = make_float4(o.x, o.y, o.z, d.x);
+		Q.od1[slot] = make_float4(d.y, d.z, 1.0f, 1.0f);
+		Q.tl[slot] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+		Q.meta[slot] = make_uint4(rng, g, (uint32_t) k | (depth << 16), 0u);
+	}
+}
+
+__global__ void wfAdvanceGenerate(WfParams P, int cur) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	const uint32_t n = genCount(P, cur);
+	P.counters[kCount0 + cur] += n;
+	P.counters[kNextPath] += n;
+	P.counters[kWorkTrace] = 0;
+	P.counters[kWorkShade] = 0;
+}
+
+__global__ void wfAdvanceShade(WfParams P, int cur) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	P.counters[kCount0 + cur] = 0;
+}
+
+// Next chunk of 64 items for this wave from a work counter.
+__device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
+	uint32_t base = 0;
+	if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64u);
+	return __shfl(base, 0, 64);
+}
+
+// Closest hit of every live path in pool `cur` (persistent).
+template <bool kMetrics, int kCap, int kShort>
+__global__ __launch_bounds__(kBlock) void wfTrace(WfParams P, int cur) {
+	__shared__ StackLdsT<kShort> stackLds;
+	const LdsStack lds = laneStack(stackLds);
+	const uint32_t count = P.counters[kCount0 + cur];
+	const WfPool Q = P.pool[cur];
+	const int lane = threadIdx.x & 63;
+	Counters<kMetrics> c;
+	c.zero();
+	for (;;) {
+		const uint32_t base = fetchChunk(P.counters + kWorkTrace);
+		if (base >= count) break;
+		const uint32_t i = base + lane;
+		if (i < count) {
+			const uint4 meta = Q.meta[i];
+			if ((meta.z >> 16) == kDead) {
+				P.hitKI[i] = kMissKind << 29;
+				continue;
+			}
+			const float4 a = Q.od0[i], b = Q.od1[i];
+			const f3 o = mk(a.x, a.y, a.z), d = mk(a.w, b.x, b.y);
+			const SceneRange &R = P.scenes[meta.z & 0xFFFFu].R;
+			c.add(0, 1);
+			Best best;
+			const bool found = traceClosest<kMetrics, kCap, kShort>(P.S, R, o, d, best, c, lds);
+			P.hitT[i] = best.t;
+			P.hitKI[i] = found ? ((uint32_t) best.kind << 29) | (uint32_t) best.index : (kMissKind << 29);
+		}
+	}
+	if (P.stats) flushCounters<kMetrics>(c, P.stats);
+}
+
+// Shading and compaction of every live path of pool `cur` into pool cur ^ 1.
+__global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
+	const uint32_t count = P.counters[kCount0 + cur];
+	const WfPool Q = P.pool[cur], D = P.pool[cur ^ 1];
+	const int lane = threadIdx.x & 63;
+	for (;;) {
+		const uint32_t base = fetchChunk(P.counters + kWorkShade);
+		if (base >= count) break;
+		const uint32_t i = base + lane;
+		bool alive = false;
+		float4 od0, od1, tl;
+		uint4 meta;
+		if (i < count) {
+			meta = Q.meta[i];
+			const uint32_t depth = meta.z >> 16;
+			if (depth != kDead) {
+				od0 = Q.od0[i];
+				od1 = Q.od1[i];
+				tl = Q.tl[i];
+				f3 o = mk(od0.x, od0.y, od0.z), d = mk(od0.w, od1.x, od1.y);
+				f3 T = mk(od1.z, od1.w, tl.x), L = mk(tl.y, tl.z, tl.w);
+				uint32_t rng = meta.x;
+				const WfScene &V = P.scenes[meta.z & 0xFFFFu];
+				const uint32_t ki = P.hitKI[i];
+				const int kind = (int) (ki >> 29);
+				Best best;
+				best.t = P.hitT[i];
+				best.kind = kind == (int) kMissKind ? -1 : kind;
+				best.index = (int) (ki & 0x1FFFFFFFu);
+				best.rank = 0;
+				const bool found = kind != (int) kMissKind;
+				alive = shadeSegment(P.S, V.R, found, best, (int) depth, V.maxDepth, V.rrStart, o, d, T, L, rng) &&
+				        (int) depth + 1 < V.maxDepth;
+				if (alive) {
+					od0 = make_float4(o.x, o.y, o.z, d.x);
+					od1 = make_float4(d.y, d.z, T.x, T.y);
+					tl = make_float4(T.z, L.x, L.y, L.z);
+					meta.x = rng;
+					meta.z = (meta.z & 0xFFFFu) | ((depth + 1) << 16);
+				} else {
+					float *r = P.results + 3 * (size_t) meta.y;
+					r[0] = L.x;
+					r[1] = L.y;
+					r[2] = L.z;
+				}
+			}
+		}
+		// wave64 compaction: one atomic per wave, prefix of the ballot per lane
+		const unsigned long long mask = __ballot(alive);
+		const uint32_t n = (uint32_t) __popcll(mask);
+		uint32_t dst = 0;
+		if (lane == 0 && n) dst = atomicAdd(P.counters + kCount0 + (cur ^ 1), n);
+		dst = __shfl(dst, 0, 64);
+		if (alive) {
+			const uint32_t slot = dst + (uint32_t) __popcll(mask & ((1ull << lane) - 1ull));
+			D.od0[slot] = od0;
+			D.od1[slot] = od1;
+			D.tl[slot] = tl;
+			D.meta[slot] = meta;
+		}
+	}
+}
+
+// Per-pixel sum of its samples in sample order with the megakernel's last-
+// sample normalization (GpuKernelSources.java:1284-1293).
+__global__ __launch_bounds__(kBlock) void wfResolve(WfParams P) {
+	for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < P.totalPixels; p += gridDim.x * kBlock) {
+		const WfScene &V = P.scenes[sceneOfPixel(P, p)];
+		const uint32_t pixel = p - V.pixelBase;
+		const float *r = P.results + 3 * ((size_t) V.pathBase + (size_t) pixel * V.spp);
+		f3 acc = mk(0.0f, 0.0f, 0.0f);
+		for (int s = 0; s < V.spp; s++) {
+			const f3 col = mk(r[3 * s], r[3 * s + 1], r[3 * s + 2]);
+			acc = s == 0 ? col : acc + col;
+			if (s == V.spp - 1) acc = acc * (1.0f / (float) V.spp);
+		}
+		P.images[3 * (size_t) p] = acc.x;
+		P.images[3 * (size_t) p + 1] = acc.y;
+		P.images[3 * (size_t) p + 2] = acc.z;
+	}
+}
+
+// Trace kernels compiled ahead of time: stack storage 64 or 256 entries, of
+// which 4 or 8 are held in LDS (the autotuned LDS split).
+using TraceFn = void (*)(WfParams, int);
+template <bool M>
+TraceFn traceFor(int cap, int ldsEntries) {
+	if (ldsEntries <= 4) return cap <= 64 ? wfTrace<M, 64, 4> : wfTrace<M, 256, 4>;
+	return cap <= 64 ? wfTrace<M, 64, 8> : wfTrace<M, 256, 8>;
+}
+
+}  // namespace mrad
+
+using namespace mrad;
+
+struct mrad_batch {
+	std::mutex mu;
+	mrad_wavefront_config cfg{};
+	bool available = false;
+	std::string unavailableReason;
+	hipStream_t stream = nullptr;
+	hipEvent_t evStart = nullptr, evEnd = nullptr;
+	int cuCount = 0;
+	HostArena host;
+	std::vector<WfScene> scenes;
+	bool dirty = true;
+	DevArena arena;
+	DevBuf sceneTable, pool[2][4], hitT, hitKI, results, images, counters, stats;
+	hipGraphExec_t graph = nullptr;
+	std::vector<hipEvent_t> stepEvents;
+	uint32_t totalPaths = 0, totalPixels = 0;
+	uint32_t *hostCounters = nullptr;    // pinned
+	mrad_batch_stats last{};
+};
+
+namespace {
+
+int requireBatch(mrad_batch *b) {
+	if (!b) return set_error(MRAD_ERR_INVALID_ARGUMENT, "batch handle is null");
+	if (!b->available) return set_error(MRAD_ERR_UNAVAILABLE, "HIP batch renderer is unavailable: " + b->unavailableReason);
+	return MRAD_OK;
+}
+
+void dropGraph(mrad_batch *b) {
+	if (b->graph) (void) hipGraphExecDestroy(b->graph);
+	b->graph = nullptr;
+}
+
+// Uploads the arena and scene table and sizes the pools for the current scenes.
+int prepare(mrad_batch *b) {
+	if (!b->dirty) return MRAD_OK;
+	dropGraph(b);
+	MRAD_TRY(b->arena.upload(b->stream, b->host));
+	MRAD_TRY(upload(b->stream, b->sceneTable, b->scenes));
+	const uint32_t cap = (uint32_t) b->cfg.queue_capacity;
+	for (int q = 0; q < 2; q++)
+		for (int k = 0; k < 4; k++) MRAD_TRY(ensure(b->pool[q][k], (size_t) cap * 16));
+	MRAD_TRY(ensure(b->hitT, (size_t) cap * 4));
+	MRAD_TRY(ensure(b->hitKI, (size_t) cap * 4));
+	MRAD_TRY(ensure(b->results, (size_t) b->totalPaths * 12));
+	MRAD_TRY(ensure(b->images, (size_t) b->totalPixels * 12));
+	MRAD_TRY(ensure(b->counters, kCounterWords * 4));
+	MRAD_TRY(ensure(b->stats, MRAD_STAT_COUNT * 8));
+	MRAD_HIP(hipStreamSynchronize(b->stream));
+	b->dirty = false;
+	return MRAD_OK;
+}
+
+WfParams params(mrad_batch *b) {
+	WfParams P{};
+	P.S = b->arena.view;
+	P.scenes = b->sceneTable.as<const WfScene>();
+	P.sceneCount = (int) b->scenes.size();
+	for (int q = 0; q < 2; q++) {
+		P.pool[q].od0 = b->pool[q][0].as<float4>();
+		P.pool[q].od1 = b->pool[q][1].as<float4>();
+		P.pool[q].tl = b->pool[q][2].as<float4>();
+		P.pool[q].meta = b->pool[q][3].as<uint4>();
+	}
+	P.hitT = b->hitT.as<float>();
+	P.hitKI = b->hitKI.as<uint32_t>();
+	P.results = b->results.as<float>();
+	P.images = b->images.as<float>();
+	P.counters = b->counters.as<uint32_t>();
+	P.totalPaths = b->totalPaths;
+	P.totalPixels = b->totalPixels;
+	P.capacity = (uint32_t) b->cfg.queue_capacity;
+	P.stats = b->cfg.collect_metrics ? b->stats.as<unsigned long long>() : nullptr;
+	return P;
+}
+
+// One bounce step on pool `cur`: generate, trace, shade into cur ^ 1.  With
+// `ev` (4 events) the kernel boundaries are recorded for per-kernel timing.
+int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = nullptr) {
+	const int persistentBlocks = std::max(1, b->cuCount * b->cfg.waves_per_cu / kWaves);
+	const int genBlocks = std::max(1, std::min(8192, (int) ((b->cfg.queue_capacity + kBlock - 1) / kBlock)));
+	if (ev) MRAD_HIP(hipEventRecord(ev[0], b->stream));
+	hipLaunchKernelGGL(wfGenerate, dim3(genBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	hipLaunchKernelGGL(wfAdvanceGenerate, dim3(1), dim3(64), 0, b->stream, P, cur);
+	if (ev) MRAD_HIP(hipEventRecord(ev[1], b->stream));
+	const TraceFn trace = b->cfg.collect_metrics ? traceFor<true>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries)
+	                                             : traceFor<false>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries);
+	hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
+	hipLaunchKernelGGL(wfShade, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	hipLaunchKernelGGL(wfAdvanceShade, dim3(1), dim3(64), 0, b->stream, P, cur);
+	if (ev) MRAD_HIP(hipEventRecord(ev[3], b->stream));
+	MRAD_HIP(hipGetLastError());
+	return MRAD_OK;
+}
+
+// Adds the kernel intervals of one timed step to the stats.
+int accumulateStep(mrad_batch *b, hipEvent_t *ev) {
+	float g = 0.0f, tr = 0.0f, sh = 0.0f;
+	MRAD_HIP(hipEventSynchronize(ev[3]));
+	MRAD_HIP(hipEventElapsedTime(&g, ev[0], ev[1]));
+	MRAD_HIP(hipEventElapsedTime(&tr, ev[1], ev[2]));
+	MRAD_HIP(hipEventElapsedTime(&sh, ev[2], ev[3]));
+	b->last.generate_nanos += (int64_t) (g * 1.0e6f);
+	b->last.trace_nanos += (int64_t) (tr * 1.0e6f);
+	b->last.shade_nanos += (int64_t) (sh * 1.0e6f);
+	return MRAD_OK;
+}
+
+int buildGraph(mrad_batch *b, const WfParams &P) {
+	if (b->graph) return MRAD_OK;
+	hipGraph_t g = nullptr;
+	MRAD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+	int rc = enqueueStep(b, P, 0);
+	if (rc == MRAD_OK) rc = enqueueStep(b, P, 1);
+	hipError_t e = hipStreamEndCapture(b->stream, &g);
+	if (rc != MRAD_OK) {
+		if (g) (void) hipGraphDestroy(g);
+		return rc;
+	}
+	if (e != hipSuccess) return set_error(MRAD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+	e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+	(void) hipGraphDestroy(g);
+	if (e != hipSuccess) {
+		b->graph = nullptr;
+		return set_error(MRAD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+	}
+	return MRAD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out) {
+	clear_error();
+	if (!config || !out) return set_error(MRAD_ERR_INVALID_ARGUMENT, "config and output handle are required");
+	*out = nullptr;
+	mrad_wavefront_config c = *config;
+	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
+	if (c.lds_stack_entries == 0) c.lds_stack_entries = 8;
+	if (c.lds_stack_entries != 4 && c.lds_stack_entries != 8)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "LDS stack entries must be 4 or 8");
+	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28))
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "queue capacity must be in [64, 2^28]");
+	if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return set_error(MRAD_ERR_INVALID_ARGUMENT, "waves per CU must be in [1, 32]");
+	if (c.bvh_stack_size < 8 || c.bvh_stack_size > 256)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH stack size must be in [8, 256]");
+	c.queue_capacity = (c.queue_capacity + 63) & ~63;
+	mrad_batch *b = new (std::nothrow) mrad_batch();
+	if (!b) return set_error(MRAD_ERR_OUT_OF_MEMORY, "batch allocation failed");
+	b->cfg = c;
+	int count = 0;
+	hipError_t e = hipGetDeviceCount(&count);
+	if (e != hipSuccess || count <= 0) {
+		b->unavailableReason = e != hipSuccess ? hipGetErrorString(e) : "no HIP device";
+	} else if (c.device < 0 || c.device >= count) {
+		b->unavailableReason = "device ordinal out of range";
+	} else if ((e = hipSetDevice(c.device)) != hipSuccess ||
+	           (e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+	           (e = hipEventCreate(&b->evStart)) != hipSuccess || (e = hipEventCreate(&b->evEnd)) != hipSuccess ||
+	           (e = hipDeviceGetAttribute(&b->cuCount, hipDeviceAttributeMultiprocessorCount, c.device)) != hipSuccess ||
+	           (e = hipHostMalloc((void **) &b->hostCounters, kCounterWords * 4, hipHostMallocDefault)) != hipSuccess) {
+		b->unavailableReason = hipGetErrorString(e);
+	} else {
+		b->available = true;
+	}
+	*out = b;
+	return MRAD_OK;
+}
+
+void mrad_batch_destroy(mrad_batch *b) {
+	if (!b) return;
+	{
+		std::lock_guard<std::mutex> g(b->mu);
+		if (b->available) {
+			(void) hipSetDevice(b->cfg.device);
+			if (b->stream) (void) hipStreamSynchronize(b->stream);
+			dropGraph(b);
+			b->arena.release();
+			for (int q = 0; q < 2; q++)
+				for (int k = 0; k < 4; k++) b->pool[q][k].release();
+			for (DevBuf *x : {&b->sceneTable, &b->hitT, &b->hitKI, &b->results, &b->images, &b->counters, &b->stats})
+				x->release();
+			if (b->hostCounters) (void) hipHostFree(b->hostCounters);
+			for (hipEvent_t e : b->stepEvents) (void) hipEventDestroy(e);
+			if (b->evStart) (void) hipEventDestroy(b->evStart);
+			if (b->evEnd) (void) hipEventDestroy(b->evEnd);
+			if (b->stream) (void) hipStreamDestroy(b->stream);
+		}
+	}
+	delete b;
+}
+
+int mrad_batch_add_scene(mrad_batch *b, const mrad_scene_arrays *a, const mrad_view *v, int *index) {
+	clear_error();
+	if (!b || !a || !v) return set_error(MRAD_ERR_INVALID_ARGUMENT, "batch, scene and view are required");
+	if (v->width < 1 || v->height < 1 || v->samples < 1)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "view width, height and samples must be positive");
+	MRAD_TRY(mrad_validate_full_path_depth(v->max_depth));
+	if (v->rr_start < -1) return set_error(MRAD_ERR_INVALID_ARGUMENT, "roulette start depth must be >= -1");
+	std::lock_guard<std::mutex> g(b->mu);
+	if (b->scenes.size() >= 0xFFFF) return set_error(MRAD_ERR_INVALID_ARGUMENT, "at most 65535 scenes per batch");
+	const uint64_t pixels = (uint64_t) v->width * v->height;
+	const uint64_t paths = pixels * (uint64_t) v->samples;
+	if ((uint64_t) b->totalPaths + paths >= 0xFFFFFFFFull || (uint64_t) b->totalPixels + pixels >= 0x7FFFFFFFull)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "batch exceeds 2^32 paths");
+	WfScene s{};
+	MRAD_TRY(b->host.append(a, b->cfg.linear_scan == 0, b->cfg.bvh_stack_size, s.R));
+	std::memcpy(s.cam, v->camera, sizeof s.cam);
+	s.width = v->width;
+	s.height = v->height;
+	s.spp = v->samples;
+	s.maxDepth = v->max_depth;
+	s.rrStart = v->rr_start;
+	s.frameIndex = v->frame_index;
+	s.frameSeed = v->frame_seed;
+	s.pathBase = b->totalPaths;
+	s.pixelBase = b->totalPixels;
+	b->totalPaths += (uint32_t) paths;
+	b->totalPixels += (uint32_t) pixels;
+	b->scenes.push_back(s);
+	b->dirty = true;
+	if (index) *index = (int) b->scenes.size() - 1;
+	return MRAD_OK;
+}
+
+int mrad_batch_render(mrad_batch *b) {
+	clear_error();
+	MRAD_TRY(requireBatch(b));
+	std::lock_guard<std::mutex> g(b->mu);
+	if (b->scenes.empty()) return set_error(MRAD_ERR_ILLEGAL_STATE, "the batch holds no scene");
+	MRAD_HIP(hipSetDevice(b->cfg.device));
+	const int64_t t0 = nowNanos();
+	const bool wasDirty = b->dirty;
+	MRAD_TRY(prepare(b));
+	b->last.upload_nanos = wasDirty ? nowNanos() - t0 : 0;
+	const WfParams P = params(b);
+	const bool timed = b->cfg.time_kernels != 0;
+	if (b->cfg.use_graph && !timed) MRAD_TRY(buildGraph(b, P));
+	b->last.trace_nanos = b->last.shade_nanos = b->last.generate_nanos = 0;
+	while (timed && b->stepEvents.size() < 4) {
+		hipEvent_t e;
+		MRAD_HIP(hipEventCreate(&e));
+		b->stepEvents.push_back(e);
+	}
+	// counters: zero in host memory, then one copy (no memset node is recorded)
+	std::memset(b->hostCounters, 0, kCounterWords * 4);
+	MRAD_HIP(hipMemcpyAsync(b->counters.p, b->hostCounters, kCounterWords * 4, hipMemcpyHostToDevice, b->stream));
+	if (b->cfg.collect_metrics) MRAD_HIP(hipMemsetAsync(b->stats.p, 0, MRAD_STAT_COUNT * 8, b->stream));
+	MRAD_HIP(hipEventRecord(b->evStart, b->stream));
+	int64_t steps = 0;
+	for (;;) {
+		if (timed) {
+			for (int cur = 0; cur < 2; cur++) {
+				MRAD_TRY(enqueueStep(b, P, cur, b->stepEvents.data()));
+				MRAD_TRY(accumulateStep(b, b->stepEvents.data()));
+			}
+		} else if (b->graph) {
+			MRAD_HIP(hipGraphLaunch(b->graph, b->stream));
+		} else {
+			MRAD_TRY(enqueueStep(b, P, 0));
+			MRAD_TRY(enqueueStep(b, P, 1));
+		}
+		steps += 2;
+		MRAD_HIP(hipMemcpyAsync(b->hostCounters, b->counters.p, kCounterWords * 4, hipMemcpyDeviceToHost, b->stream));
+		MRAD_HIP(hipStreamSynchronize(b->stream));
+		const uint32_t *c = b->hostCounters;
+		if (c[kNextPath] >= b->totalPaths && c[kCount0] == 0 && c[kCount1] == 0) break;
+		if (steps > (int64_t) 64 + 4 * ((int64_t) b->totalPaths / 64 + 1) * 33)
+			return set_error(MRAD_ERR_ILLEGAL_STATE, "wavefront did not drain");
+	}
+	const int resolveBlocks = std::max(1, std::min(16384, (int) ((b->totalPixels + kBlock - 1) / kBlock)));
+	hipLaunchKernelGGL(wfResolve, dim3(resolveBlocks), dim3(kBlock), 0, b->stream, P);
+	MRAD_HIP(hipGetLastError());
+	MRAD_HIP(hipEventRecord(b->evEnd, b->stream));
+	MRAD_HIP(hipEventSynchronize(b->evEnd));
+	float ms = 0.0f;
+	MRAD_HIP(hipEventElapsedTime(&ms, b->evStart, b->evEnd));
+	b->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
+	b->last.steps = steps;
+	b->last.paths = b->totalPaths;
+	if (b->cfg.collect_metrics) MRAD_HIP(hipMemcpy(b->last.counters, b->stats.p, sizeof b->last.counters, hipMemcpyDeviceToHost));
+	else std::memset(b->last.counters, 0, sizeof b->last.counters);
+	b->last.total_nanos = nowNanos() - t0;
+	return MRAD_OK;
+}
+
+int mrad_batch_copy_image(mrad_batch *b, int scene, float *rgb) {
+	clear_error();
+	MRAD_TRY(requireBatch(b));
+	if (!rgb) return set_error(MRAD_ERR_INVALID_ARGUMENT, "RGB output must not be null");
+	std::lock_guard<std::mutex> g(b->mu);
+	if (scene < 0 || scene >= (int) b->scenes.size()) return set_error(MRAD_ERR_INVALID_ARGUMENT, "scene index out of range");
+	if (b->dirty || !b->images.p) return set_error(MRAD_ERR_ILLEGAL_STATE, "the batch has not been rendered");
+	const WfScene &s = b->scenes[scene];
+	MRAD_HIP(hipSetDevice(b->cfg.device));
+	MRAD_HIP(hipMemcpy(rgb, b->images.as<float>() + 3 * (size_t) s.pixelBase, (size_t) s.width * s.height * 12,
+	                   hipMemcpyDeviceToHost));
+	return MRAD_OK;
+}
+
+int mrad_batch_get_stats(mrad_batch *b, mrad_batch_stats *out) {
+	clear_error();
+	if (!b || !out) return set_error(MRAD_ERR_INVALID_ARGUMENT, "batch and output are required");
+	std::lock_guard<std::mutex> g(b->mu);
+	*out = b->last;
+	return MRAD_OK;
+}
+
+}  // extern "C"

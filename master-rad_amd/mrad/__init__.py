@@ -7,3 +7,4 @@ from ._native import MradError, InvalidArgument, IllegalState, STAT_NAMES  # noq
 from .scene import BvhBuildConfig, GpuScene, material_array  # noqa: F401
 from .tracer import (GpuCamera, GpuRayTracer, HeadlessGpuRenderer, TraceReplayResult,  # noqa: F401
                      frame_seed, mix64, validate_full_path_depth)
+from .batch import BatchRenderer, WavefrontConfig  # noqa: F401

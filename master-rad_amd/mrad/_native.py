@@ -102,6 +102,36 @@ SIGNATURES = {
 }
 
 
+class View(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples", C.c_int), ("max_depth", C.c_int),
+                ("rr_start", C.c_int), ("frame_index", C.c_int), ("frame_seed", C.c_uint64),
+                ("camera", C.c_float * 12)]
+
+
+class WavefrontConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("queue_capacity", C.c_int), ("waves_per_cu", C.c_int),
+                ("bvh_stack_size", C.c_int), ("linear_scan", C.c_int), ("collect_metrics", C.c_int),
+                ("use_graph", C.c_int), ("time_kernels", C.c_int), ("lds_stack_entries", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("total_nanos", C.c_int64), ("kernel_nanos", C.c_int64), ("upload_nanos", C.c_int64),
+                ("steps", C.c_int64), ("paths", C.c_uint64), ("counters", C.c_uint64 * STAT_COUNT),
+                ("trace_nanos", C.c_int64), ("shade_nanos", C.c_int64), ("generate_nanos", C.c_int64)]
+
+
+SIGNATURES.update({
+    "mrad_batch_create": (C.c_int, [C.POINTER(WavefrontConfig), C.POINTER(C.c_void_p)]),
+    "mrad_batch_destroy": (None, [C.c_void_p]),
+    "mrad_batch_add_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneArrays), C.POINTER(View), C.POINTER(C.c_int)]),
+    "mrad_batch_render": (C.c_int, [C.c_void_p]),
+    "mrad_batch_copy_image": (C.c_int, [C.c_void_p, C.c_int, fp]),
+    "mrad_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "mrad_backward": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64, fp, fp, C.c_int]),
+    "mrad_update_materials": (C.c_int, [C.c_void_p, fp, C.c_int]),
+})
+
+
 class MradError(RuntimeError):
     """Base of the status-code exceptions raised by the binding."""
 

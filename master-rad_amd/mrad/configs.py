@@ -211,8 +211,19 @@ def batch_specs(count: int = 64, max_prims: int = 1_000_000) -> list[dict]:
 
 def batch_scene(spec: dict) -> Workload:
     sc = field_scene(spec["seed"], spec["prims"])
+    sc.bvh_stats()          # packs and builds the BVH (the native build releases the GIL)
     return Workload(f"batch-{spec['index']}", sc, field_camera(), spec["width"], spec["height"], spec["spp"],
                     spec["max_depth"], spec["rr_start"], frame_seed=spec["seed"] & 0xFFFFFFFF)
+
+
+def batch(count: int = 64, threads: int = 16, max_prims: int = 1_000_000) -> list[Workload]:
+    """The 64-scene batch, scenes generated and their BVHs built on a thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+    specs = batch_specs(count, max_prims)
+    order = sorted(range(count), key=lambda i: -specs[i]["prims"])   # largest first for balance
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        built = dict(zip(order, ex.map(lambda i: batch_scene(specs[i]), order)))
+    return [built[i] for i in range(count)]
 
 
 def diff(n: int = 10_000) -> Workload:

@@ -58,6 +58,9 @@ def _lib() -> C.CDLL:
     lib.or64_render.restype = C.c_int
     lib.or64_render.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    lib.or_adjoint.restype = C.c_int
+    lib.or_adjoint.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                               C.c_int, fp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.or64_closest.restype = C.c_int
     lib.or64_closest.argtypes = [C.POINTER(OrScene), fp, C.POINTER(C.c_double), ip, ip, C.POINTER(C.c_double)]
     return lib
@@ -160,6 +163,20 @@ def render64(scene: Scene, camera: np.ndarray, width: int, height: int, max_dept
                          var.ctypes.data_as(C.POINTER(C.c_double)) if variance else None, threads)
     assert rc == 0
     return (mean, var) if variance else mean
+
+
+def adjoint(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, frame_index: int,
+            frame_seed: int, spp: int, grad_image: np.ndarray, primitive_count: int, rr_start: int = -1):
+    """Path-replay adjoint with FP32 paths and FP64 sums: (grad_albedo, grad_emission) per primitive."""
+    cam = np.ascontiguousarray(camera, np.float32)
+    g = np.ascontiguousarray(grad_image, np.float32).reshape(-1)
+    ga = np.zeros(primitive_count * 3, np.float64)
+    ge = np.zeros(primitive_count * 3, np.float64)
+    rc = lib.or_adjoint(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, max_depth, rr_start, frame_index,
+                        frame_seed & ((1 << 64) - 1), spp, g.ctypes.data_as(fp),
+                        ga.ctypes.data_as(C.POINTER(C.c_double)), ge.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    return ga.reshape(-1, 3), ge.reshape(-1, 3)
 
 
 def closest64(scene: Scene, ray6):

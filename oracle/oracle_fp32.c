@@ -759,6 +759,119 @@ int or_render_pixels(const OrScene *s, const float *cam, int width, int height, 
 	return 0;
 }
 
+/* Reference global order of the closest hit (GpuKernelSources.java:784-799, :1017). */
+static int hitOrder(const Hit *h) { return h->order; }
+
+/* Path-replay adjoint of one path (no counterpart in the reference).  The
+ * path is re-traced with tracePath's arithmetic and random numbers; its
+ * vertices are kept (throughput T_i, lobe weight w_i, diffuse-albedo factor
+ * g_i = 1/(p_lobe q_roulette) or 0 for a non-diffuse lobe, emission Le_i,
+ * primitive order) and swept backwards:
+ *   dLoss/dLe_i     += delta * T_i
+ *   dLoss/dalbedo_i += delta * T_i * g_i * Lin_i
+ *   Lin_{i-1}        = Le_i + w_i * Lin_i
+ * with Lin of the last vertex = background if its scattered ray missed, else 0.
+ * Lobe-selection and roulette probabilities are treated as constants.  Sums in
+ * double.  Returns the path radiance (float, as tracePath). */
+static v3 adjointPath(const OrScene *s, v3 o, v3 d, int maxDepth, int rrStart, uint32_t *rng, const double *delta,
+		double *gAlb, double *gEmit) {
+	enum { MAXV = 32 };
+	v3 Tv[MAXV], Wv[MAXV], Lev[MAXV];
+	double gv[MAXV];
+	int prim[MAXV];
+	int n = 0;
+	v3 T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
+	v3 bg = mk(s->bg[0], s->bg[1], s->bg[2]);
+	v3 tail = mk(0.0f, 0.0f, 0.0f);
+	for (int depth = 0; depth < maxDepth; ++depth) {
+		Hit h;
+		int hf = traceClosest(s, o, d, &h, NULL);
+		if (!hf) { L = vadd(L, vmul(T, bg)); tail = bg; break; }
+		L = vadd(L, vmul(T, h.m.emittance));
+		Tv[n] = T; Lev[n] = h.m.emittance; prim[n] = hitOrder(&h);
+		Wv[n] = mk(0.0f, 0.0f, 0.0f); gv[n] = 0.0;
+		n++;
+		v3 nrm = h.normal, in = vneg(d);
+		v3 cD = h.m.diffuse, cR = h.m.reflective, cT = h.m.refractive;
+		float wD = luminance(cD), wR = luminance(cR), wT = luminance(cT);
+		float sum = wD + wR + wT;
+		if (sum <= 0.0f) break;
+		float xi = rand01(rng) * sum;
+		v3 nd, wc;
+		float p;
+		int diffuse = 0;
+		if (xi < wD) {
+			p = wD / sum; diffuse = 1;
+			nd = vadd(sampleUnitSphere(rng), nrm);
+			wc = vdivs(cD, p);
+		} else if (xi < wD + wR) {
+			p = wR / sum;
+			nd = reflect_dir(nrm, in);
+			wc = vdivs(cR, p);
+		} else {
+			p = wT / sum;
+			v3 rd;
+			refract_dir(in, nrm, h.m.ior, &rd);
+			nd = rd;
+			wc = vdivs(cT, p);
+		}
+		o = vadd(h.pos, vmuls(nd, EPSILON));
+		T = vmul(T, wc);
+		d = nd;
+		double scale = diffuse ? 1.0 / (double) p : 0.0;
+		double q = 1.0;
+		if (rrStart >= 0 && depth + 1 >= rrStart && depth + 1 < maxDepth) {
+			float m = fmaxf(T.x, fmaxf(T.y, T.z));
+			float qf = fminf(fmaxf(m, 0.05f), 0.95f);
+			if (rand01(rng) >= qf) { Wv[n - 1] = wc; gv[n - 1] = scale; break; }
+			T = vmuls(T, 1.0f / qf);
+			q = qf;
+		}
+		Wv[n - 1] = mk((float) (wc.x / q), (float) (wc.y / q), (float) (wc.z / q));
+		gv[n - 1] = scale / q;
+		if (depth + 1 == maxDepth) break;
+	}
+	double lin[3] = {tail.x, tail.y, tail.z};
+	/* a path killed by roulette or ended by depth after its last scatter has no incoming radiance */
+	for (int i = n - 1; i >= 0; i--) {
+		const double t[3] = {Tv[i].x, Tv[i].y, Tv[i].z};
+		const double le[3] = {Lev[i].x, Lev[i].y, Lev[i].z};
+		const double w[3] = {Wv[i].x, Wv[i].y, Wv[i].z};
+		for (int c = 0; c < 3; c++) {
+			gEmit[3 * prim[i] + c] += delta[c] * t[c];
+			gAlb[3 * prim[i] + c] += delta[c] * t[c] * gv[i] * lin[c];
+			lin[c] = le[c] + w[c] * lin[c];
+		}
+	}
+	return L;
+}
+
+/* Adjoint of the whole frame: delta = dLoss/dImage (W*H*3 floats); gradients
+ * per primitive in the reference's global order (grad arrays nPrim x 3). */
+int or_adjoint(const OrScene *s, const float *cam, int width, int height, int maxDepth, int rrStart, int frameIndex,
+		uint64_t frameSeed, int spp, const float *gradImage, double *gAlb, double *gEmit) {
+	if (!s || !cam || !gradImage || !gAlb || !gEmit || spp < 1 || maxDepth > 32) return -1;
+	v3 pos = mk(cam[0], cam[1], cam[2]), fwd = mk(cam[3], cam[4], cam[5]);
+	v3 right = mk(cam[6], cam[7], cam[8]), up = mk(cam[9], cam[10], cam[11]);
+	for (int y = 0; y < height; y++)
+		for (int x = 0; x < width; x++) {
+			uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
+			double delta[3];
+			for (int c = 0; c < 3; c++) delta[c] = (double) gradImage[3 * pixel + c] / (double) spp;
+			for (int so = 0; so < spp; so++) {
+				uint32_t rng = or_path_seed(frameSeed, frameIndex, pixel, so);
+				float jx = rand01(&rng) - 0.5f;
+				float jy = rand01(&rng) - 0.5f;
+				float aspect = (float) width / (float) height;
+				float u = ((((float) x + 0.5f) + jx) * (2.0f / (float) width) - 1.0f) * aspect;
+				float v = 1.0f - ((((float) y + 0.5f) + jy) * (2.0f / (float) height));
+				v3 dir = vadd(vadd(fwd, vmuls(right, u)), vmuls(up, v));
+				adjointPath(s, pos, dir, maxDepth, rrStart, &rng, delta, gAlb, gEmit);
+			}
+		}
+	return 0;
+}
+
 /* traceReplayKernel body, GpuKernelSources.java:1338-1376, with the per-ray
  * counters it computes kept per ray (perRay: n x 14 u32, may be NULL). */
 int or_trace_replay(const OrScene *s, const float *rays, int n, int *flags, float *dist, int *order,
