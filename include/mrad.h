@@ -227,6 +227,7 @@ typedef struct {
 	int use_graph;           /* capture the bounce steps as a hipGraph */
 	int time_kernels;        /* time each kernel with HIP events (no graph; for measurement) */
 	int lds_stack_entries;   /* traversal-stack entries per lane held in LDS: 4 or 8 (0 = 8) */
+	int leaf_regroup;        /* 1 = regroup each wave's leaf tests by primitive class */
 } mrad_wavefront_config;
 
 typedef struct {
@@ -245,6 +246,9 @@ void mrad_batch_destroy(mrad_batch *batch);
 int mrad_batch_add_scene(mrad_batch *batch, const mrad_scene_arrays *scene, const mrad_view *view, int *index_out);
 /* Renders every scene of the batch; images stay on the device. */
 int mrad_batch_render(mrad_batch *batch);
+/* New launch parameters for a loaded batch (the autotuner's knob); device,
+ * stack size and traversal mode stay as created. */
+int mrad_batch_configure(mrad_batch *batch, const mrad_wavefront_config *config);
 /* Normalized RGB image (width*height*3 floats) of one scene. */
 int mrad_batch_copy_image(mrad_batch *batch, int scene, float *rgb_out);
 int mrad_batch_get_stats(mrad_batch *batch, mrad_batch_stats *out);

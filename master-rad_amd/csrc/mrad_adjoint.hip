@@ -37,8 +37,8 @@ struct AdjointParams {
 	int width, height, spp, maxDepth, rrStart, frameIndex, tilesX;
 	uint64_t frameSeed;
 	const float *gradImage;
-	float *gAlbedo;
-	float *gEmission;
+	double *gAlbedo;         // FP64 sums: the per-vertex terms change sign with the
+	double *gEmission;       // pixel residual, so FP32 sums lose the small net values
 };
 
 template <int kCap>
@@ -129,13 +129,13 @@ __global__ __launch_bounds__(kBlock) void adjointKernel(AdjointParams P) {
 			const int base = 3 * prim[i];
 			const f3 ge = delta * t;
 			const f3 ga = ge * lin * gv[i];
-			atomicAdd(P.gEmission + base, ge.x);
-			atomicAdd(P.gEmission + base + 1, ge.y);
-			atomicAdd(P.gEmission + base + 2, ge.z);
+			atomicAdd(P.gEmission + base, (double) ge.x);
+			atomicAdd(P.gEmission + base + 1, (double) ge.y);
+			atomicAdd(P.gEmission + base + 2, (double) ge.z);
 			if (gv[i] != 0.0f) {
-				atomicAdd(P.gAlbedo + base, ga.x);
-				atomicAdd(P.gAlbedo + base + 1, ga.y);
-				atomicAdd(P.gAlbedo + base + 2, ga.z);
+				atomicAdd(P.gAlbedo + base, (double) ga.x);
+				atomicAdd(P.gAlbedo + base + 1, (double) ga.y);
+				atomicAdd(P.gAlbedo + base + 2, (double) ga.z);
 			}
 			lin = Lev[i] + Wv[i] * lin;
 		}
@@ -191,11 +191,11 @@ int mrad_backward(mrad_tracer *t, const float *gradImage, const float camera[12]
 	const int W = t->cfg.width, H = t->cfg.height;
 	const size_t px = (size_t) W * H * 3, np = (size_t) primCount * 3;
 	MRAD_TRY(ensure(t->gradImage, px * 4));
-	MRAD_TRY(ensure(t->gradAlbedo, np * 4));
-	MRAD_TRY(ensure(t->gradEmission, np * 4));
+	MRAD_TRY(ensure(t->gradAlbedo, np * 8));
+	MRAD_TRY(ensure(t->gradEmission, np * 8));
 	MRAD_HIP(hipMemcpyAsync(t->gradImage.p, gradImage, px * 4, hipMemcpyHostToDevice, t->stream));
-	MRAD_HIP(hipMemsetAsync(t->gradAlbedo.p, 0, np * 4, t->stream));
-	MRAD_HIP(hipMemsetAsync(t->gradEmission.p, 0, np * 4, t->stream));
+	MRAD_HIP(hipMemsetAsync(t->gradAlbedo.p, 0, np * 8, t->stream));
+	MRAD_HIP(hipMemsetAsync(t->gradEmission.p, 0, np * 8, t->stream));
 	AdjointParams P{};
 	P.S = t->arena.view;
 	P.R = t->range;
@@ -209,8 +209,8 @@ int mrad_backward(mrad_tracer *t, const float *gradImage, const float camera[12]
 	P.frameSeed = frameSeed;
 	P.tilesX = (W + kAdjTile - 1) / kAdjTile;
 	P.gradImage = t->gradImage.as<const float>();
-	P.gAlbedo = t->gradAlbedo.as<float>();
-	P.gEmission = t->gradEmission.as<float>();
+	P.gAlbedo = t->gradAlbedo.as<double>();
+	P.gEmission = t->gradEmission.as<double>();
 	const int blocks = P.tilesX * ((H + kAdjTile - 1) / kAdjTile);
 	const int64_t t0 = nowNanos();
 	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
@@ -221,8 +221,11 @@ int mrad_backward(mrad_tracer *t, const float *gradImage, const float camera[12]
 	MRAD_HIP(hipEventSynchronize(t->evEnd));
 	float ms = 0.0f;
 	MRAD_HIP(hipEventElapsedTime(&ms, t->evStart, t->evEnd));
-	MRAD_HIP(hipMemcpy(gradAlbedo, t->gradAlbedo.p, np * 4, hipMemcpyDeviceToHost));
-	MRAD_HIP(hipMemcpy(gradEmission, t->gradEmission.p, np * 4, hipMemcpyDeviceToHost));
+	std::vector<double> host(np);
+	MRAD_HIP(hipMemcpy(host.data(), t->gradAlbedo.p, np * 8, hipMemcpyDeviceToHost));
+	for (size_t k = 0; k < np; k++) gradAlbedo[k] = (float) host[k];
+	MRAD_HIP(hipMemcpy(host.data(), t->gradEmission.p, np * 8, hipMemcpyDeviceToHost));
+	for (size_t k = 0; k < np; k++) gradEmission[k] = (float) host[k];
 	t->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
 	t->last.maximum_physical_kernel_nanos = t->last.kernel_nanos;
 	t->last.physical_launches = 1;

@@ -260,11 +260,267 @@ __global__ __launch_bounds__(kBlock) void wfResolve(WfParams P) {
 	}
 }
 
+// ------------------------------------------------ primitive-class regrouping
+//
+// Variant of wfTrace in which the leaf tests of a wave are regrouped by
+// primitive class.  Each lane traverses interior nodes until it reaches a leaf
+// (or finishes); then the refs of all lanes' current leaves are appended to a
+// per-wave task list in LDS, bucketed by class with a count pass and a prefix,
+// and the whole wave runs the sphere tasks, then the box tasks, then the
+// affine ones, 64 at a time with uniform control flow, so an expensive affine
+// test on one lane no longer serialises the cheap tests of the others.
+//
+// Every ref of a leaf is tested against the lane's best distance at leaf
+// entry and the lane's best is the minimum of (t, rank) over the accepted
+// candidates (an LDS 64-bit atomic min).  The reference tests a leaf's refs in
+// order against the running best (GpuKernelSources.java:808-841); a candidate
+// it rejects because an earlier ref of the same leaf got closer cannot be the
+// minimum, so the winner is the same, and the counters, which count tests
+// rather than acceptances, are the same.
+
+constexpr int kRegroupPerLane = 8;                       // refs per lane per round
+constexpr int kTaskCap = 64 * kRegroupPerLane;
+
+struct RegroupLds {
+	float ray[6][64];
+	unsigned long long key[64];
+	float limitT[64];
+	uint32_t taskRef[kTaskCap];
+	uint8_t taskOwner[kTaskCap];
+	uint32_t typeCount[4];
+	uint32_t typeCursor[4];
+};
+
+__device__ __forceinline__ void waveSync() {
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+}
+// (t, rank) as one unsigned 64-bit key ordered like the pair.
+__device__ __forceinline__ unsigned long long bestKey(float t, uint32_t rank) {
+	uint32_t b = __float_as_uint(t);
+	b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+	return ((unsigned long long) b << 32) | rank;
+}
+__device__ __forceinline__ float keyT(unsigned long long k) {
+	uint32_t b = (uint32_t) (k >> 32);
+	b = (b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b;
+	return __uint_as_float(b);
+}
+
+template <bool kMetrics, int kCap, int kShort>
+__global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
+	__shared__ StackLdsT<kShort> stackLds;
+	__shared__ RegroupLds regroup[kWaves];
+	const LdsStack lds = laneStack(stackLds);
+	RegroupLds &G = regroup[threadIdx.x >> 6];
+	const uint32_t count = P.counters[kCount0 + cur];
+	const WfPool Q = P.pool[cur];
+	const int lane = threadIdx.x & 63;
+	Counters<kMetrics> c;
+	c.zero();
+	for (;;) {
+		const uint32_t base = fetchChunk(P.counters + kWorkTrace);
+		if (base >= count) break;
+		const uint32_t i = base + lane;
+		bool valid = i < count;
+		uint4 meta = make_uint4(0, 0, 0, 0);
+		if (valid) {
+			meta = Q.meta[i];
+			if ((meta.z >> 16) == kDead) {
+				P.hitKI[i] = kMissKind << 29;
+				valid = false;
+			}
+		}
+		f3 o = mk(0.0f, 0.0f, 0.0f), d = mk(0.0f, 0.0f, 1.0f);
+		const SceneRange *Rp = P.scenes ? &P.scenes[valid ? (meta.z & 0xFFFFu) : 0].R : nullptr;
+		Best best;
+		best.t = 1e30f;
+		best.rank = 0xFFFFFFFFu;
+		best.kind = -1;
+		best.index = -1;
+		TravStack<kCap, kShort> st;
+		st.lds = lds;
+		int size = 0;
+		bool overflow = false, trav = false;
+		if (valid) {
+			const float4 a = Q.od0[i], b = Q.od1[i];
+			o = mk(a.x, a.y, a.z);
+			d = mk(a.w, b.x, b.y);
+			c.add(0, 1);
+			const SceneRange &R = *Rp;
+			if (R.useBvh == 0) {
+				traceLinear<kMetrics>(P.S, R, o, d, best, c);
+			} else if (R.refCount > 0 && R.rootCount > 0 && R.rootCount <= 4) {
+				trav = true;
+				for (int r = 0; r < R.rootCount; r++) {     // root insertion, as traceBvh
+					const int ri = R.roots[r];
+					if (ri < 0) { overflow = true; break; }
+					c.add(1, 1);
+					const float *rb = R.rootBounds[r];
+					float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, d, best.t);
+					if (!isfinite(e)) continue;
+					if (size >= R.stackLimit) { overflow = true; break; }
+					int pos = size;
+					while (pos > 0 && e >= st.entry(pos - 1)) {
+						st.set(pos, st.node(pos - 1), st.entry(pos - 1));
+						pos--;
+					}
+					st.set(pos, ri, e);
+					size++;
+					c.maxStack(size);
+				}
+			}
+		}
+		G.ray[0][lane] = o.x; G.ray[1][lane] = o.y; G.ray[2][lane] = o.z;
+		G.ray[3][lane] = d.x; G.ray[4][lane] = d.y; G.ray[5][lane] = d.z;
+		for (;;) {
+			// traverse interior nodes until this lane reaches a leaf or finishes
+			bool atLeaf = false;
+			int leafFirst = 0, leafCount = 0, leafKind = -1;
+			while (trav && !overflow) {
+				if (size == 0) { trav = false; break; }
+				--size;
+				const int node = st.node(size);
+				const float entry = st.entry(size);
+				const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
+				if (entry > ch) continue;
+				const float4 *rec = P.S.nodes + 4 * (size_t) node;
+				const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
+				if (nd.x < 0) {
+					c.add(11, 1);
+					c.add(nd.x == -1 ? 13 : 12, 1);
+					c.add(2, (uint32_t) nd.w);
+					atLeaf = true;
+					leafFirst = nd.z;
+					leafCount = nd.w;
+					leafKind = nd.x;
+					break;
+				}
+				c.add(10, 1);
+				const float4 a = rec[0], b = rec[1], cc = rec[2];
+				const float le = entryDistance(a.x, a.y, a.z, a.w, b.x, b.y, o, d, best.t);
+				const float re = entryDistance(b.z, b.w, cc.x, cc.y, cc.z, cc.w, o, d, best.t);
+				c.add(1, 2);
+				int nearN = nd.x, farN = nd.y;
+				float nearE = le, farE = re;
+				if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+				const int cap = Rp->stackLimit;
+				if (isfinite(farE)) {
+					if (size >= cap) { overflow = true; break; }
+					st.set(size, farN, farE);
+					size++;
+					c.maxStack(size);
+				}
+				if (isfinite(nearE)) {
+					if (size >= cap) { overflow = true; break; }
+					st.set(size, nearN, nearE);
+					size++;
+					c.maxStack(size);
+				}
+			}
+			if (__ballot(atLeaf) == 0ull) break;
+			// leaf tests of the whole wave, regrouped by primitive class
+			G.key[lane] = bestKey(best.t, best.rank);
+			G.limitT[lane] = best.t;
+			const bool mixed = leafKind > -2 || leafKind < -5;
+			const uint32_t leafType = leafKind == -2 ? kRefSphere : leafKind == -3 ? kRefBox
+			                        : leafKind == -4 ? kRefAffineSphere : kRefAffineBox;
+			for (int round = 0;; round += kRegroupPerLane) {
+				const int mine = atLeaf ? max(0, min(kRegroupPerLane, leafCount - round)) : 0;
+				if (__ballot(mine > 0) == 0ull) break;
+				if (lane < 4) G.typeCount[lane] = 0;
+				waveSync();
+				for (int k = 0; k < mine; k++) {
+					const int ri = leafFirst + round + k;
+					if (ri < Rp->refBase || ri >= Rp->refBase + Rp->refCount) continue;
+					const uint32_t ref = P.S.refs[ri];
+					const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
+					c.add(3 + typeSlot(type), 1);
+					atomicAdd(&G.typeCount[type >> 30], 1u);
+				}
+				waveSync();
+				if (lane < 4) {
+					uint32_t off = 0;
+					for (int t = 0; t < lane; t++) off += G.typeCount[t];
+					G.typeCursor[lane] = off;
+				}
+				waveSync();
+				for (int k = 0; k < mine; k++) {
+					const int ri = leafFirst + round + k;
+					if (ri < Rp->refBase || ri >= Rp->refBase + Rp->refCount) continue;
+					const uint32_t ref = P.S.refs[ri];
+					const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
+					const uint32_t slot = atomicAdd(&G.typeCursor[type >> 30], 1u);
+					G.taskRef[slot] = type | (ref & kRefIndexMask);
+					G.taskOwner[slot] = (uint8_t) lane;
+				}
+				waveSync();
+				uint32_t off = 0;
+				for (int t = 0; t < 4; t++) {
+					const uint32_t n = G.typeCount[t];
+					for (uint32_t b0 = 0; b0 < n; b0 += 64) {
+						const uint32_t j = b0 + lane;
+						if (j < n) {
+							const uint32_t ref = G.taskRef[off + j];
+							const int owner = G.taskOwner[off + j];
+							const f3 ro = mk(G.ray[0][owner], G.ray[1][owner], G.ray[2][owner]);
+							const f3 rd = mk(G.ray[3][owner], G.ray[4][owner], G.ray[5][owner]);
+							const float limit = nextafterf(G.limitT[owner], INFINITY);
+							const int index = (int) (ref & kRefIndexMask);
+							float tc = 0.0f;
+							bool found;
+							if (t == 0) found = testSphere(P.S.spheres[index], ro, rd, limit, tc);
+							else if (t == 1) found = testBox(P.S.boxes[2 * index], P.S.boxes[2 * index + 1], ro, rd, limit, tc);
+							else if (t == 2) found = testAffineSphere(P.S.affS + 3 * index, ro, rd, limit, tc);
+							else found = testAffineBox(P.S.affB + 3 * index, ro, rd, limit, tc);
+							if (found) atomicMin(&G.key[owner], bestKey(tc, ref));
+						}
+					}
+					off += n;
+				}
+				waveSync();
+			}
+			if (atLeaf) {
+				const unsigned long long k = G.key[lane];
+				const uint32_t rank = (uint32_t) k;
+				if (rank != best.rank || keyT(k) != best.t) {
+					best.t = keyT(k);
+					best.rank = rank;
+					const uint32_t type = rank & kRefTypeMask;
+					best.kind = type == kRefSphere ? kKindSphere : type == kRefBox ? kKindBox
+					          : type == kRefAffineSphere ? kKindAffineSphere : kKindAffineBox;
+					best.index = (int) (rank & kRefIndexMask);
+				}
+			}
+		}
+		if (valid) {
+			const SceneRange &R = *Rp;
+			if (overflow) {
+				c.add(8, 1);
+				for (int r = R.refBase; r < R.refBase + R.refCount; r++) {
+					const uint32_t ref = P.S.refs[r];
+					c.add(2, 1);
+					c.add(3 + typeSlot(ref & kRefTypeMask), 1);
+					testRef(P.S, ref, ref & kRefTypeMask, o, d, best);
+				}
+			}
+			c.add(2, (uint32_t) R.planeCount);
+			c.add(5, (uint32_t) R.planeCount);
+			tracePlanes(P.S, R, o, d, best);
+			P.hitT[i] = best.t;
+			P.hitKI[i] = best.kind >= 0 ? ((uint32_t) best.kind << 29) | (uint32_t) best.index : (kMissKind << 29);
+		}
+	}
+	if (P.stats) flushCounters<kMetrics>(c, P.stats);
+}
+
 // Trace kernels compiled ahead of time: stack storage 64 or 256 entries, of
-// which 4 or 8 are held in LDS (the autotuned LDS split).
+// which 4 or 8 are held in LDS (the autotuned LDS split); optionally with
+// primitive-class regrouping of the leaf tests.
 using TraceFn = void (*)(WfParams, int);
 template <bool M>
-TraceFn traceFor(int cap, int ldsEntries) {
+TraceFn traceFor(int cap, int ldsEntries, bool regroup) {
+	if (regroup) return cap <= 64 ? wfTraceRegroup<M, 64, 8> : wfTraceRegroup<M, 256, 8>;
 	if (ldsEntries <= 4) return cap <= 64 ? wfTrace<M, 64, 4> : wfTrace<M, 256, 4>;
 	return cap <= 64 ? wfTrace<M, 64, 8> : wfTrace<M, 256, 8>;
 }
@@ -358,8 +614,10 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 	hipLaunchKernelGGL(wfGenerate, dim3(genBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	hipLaunchKernelGGL(wfAdvanceGenerate, dim3(1), dim3(64), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[1], b->stream));
-	const TraceFn trace = b->cfg.collect_metrics ? traceFor<true>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries)
-	                                             : traceFor<false>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries);
+	const bool regroup = b->cfg.leaf_regroup != 0;
+	const TraceFn trace = b->cfg.collect_metrics
+	                          ? traceFor<true>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries, regroup)
+	                          : traceFor<false>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries, regroup);
 	hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
 	hipLaunchKernelGGL(wfShade, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
@@ -560,6 +818,38 @@ int mrad_batch_render(mrad_batch *b) {
 	if (b->cfg.collect_metrics) MRAD_HIP(hipMemcpy(b->last.counters, b->stats.p, sizeof b->last.counters, hipMemcpyDeviceToHost));
 	else std::memset(b->last.counters, 0, sizeof b->last.counters);
 	b->last.total_nanos = nowNanos() - t0;
+	return MRAD_OK;
+}
+
+// Changes the launch parameters of a batch (queue capacity, waves per CU, LDS
+// split, graph use, timing, metrics) keeping its scenes resident; device,
+// stack size and traversal mode must stay.  Used by the autotuner.
+int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
+	clear_error();
+	MRAD_TRY(requireBatch(b));
+	if (!config) return set_error(MRAD_ERR_INVALID_ARGUMENT, "config is required");
+	mrad_wavefront_config c = *config;
+	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
+	if (c.lds_stack_entries == 0) c.lds_stack_entries = 8;
+	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28) || c.waves_per_cu < 1 || c.waves_per_cu > 32 ||
+	    (c.lds_stack_entries != 4 && c.lds_stack_entries != 8))
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "invalid wavefront configuration");
+	std::lock_guard<std::mutex> g(b->mu);
+	if (c.device != b->cfg.device || c.bvh_stack_size != b->cfg.bvh_stack_size || c.linear_scan != b->cfg.linear_scan)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "device, stack size and traversal mode are fixed per batch");
+	c.queue_capacity = (c.queue_capacity + 63) & ~63;
+	MRAD_HIP(hipSetDevice(b->cfg.device));
+	dropGraph(b);
+	const bool grow = (size_t) c.queue_capacity * 16 > b->pool[0][0].cap;
+	b->cfg = c;
+	if (grow && !b->dirty) {
+		for (int q = 0; q < 2; q++)
+			for (int k = 0; k < 4; k++) MRAD_TRY(ensure(b->pool[q][k], (size_t) c.queue_capacity * 16));
+		MRAD_TRY(ensure(b->hitT, (size_t) c.queue_capacity * 4));
+		MRAD_TRY(ensure(b->hitKI, (size_t) c.queue_capacity * 4));
+	}
 	return MRAD_OK;
 }
 

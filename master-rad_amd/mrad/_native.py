@@ -111,7 +111,8 @@ class View(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("queue_capacity", C.c_int), ("waves_per_cu", C.c_int),
                 ("bvh_stack_size", C.c_int), ("linear_scan", C.c_int), ("collect_metrics", C.c_int),
-                ("use_graph", C.c_int), ("time_kernels", C.c_int), ("lds_stack_entries", C.c_int)]
+                ("use_graph", C.c_int), ("time_kernels", C.c_int), ("lds_stack_entries", C.c_int),
+                ("leaf_regroup", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -125,6 +126,7 @@ SIGNATURES.update({
     "mrad_batch_destroy": (None, [C.c_void_p]),
     "mrad_batch_add_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneArrays), C.POINTER(View), C.POINTER(C.c_int)]),
     "mrad_batch_render": (C.c_int, [C.c_void_p]),
+    "mrad_batch_configure": (C.c_int, [C.c_void_p, C.POINTER(WavefrontConfig)]),
     "mrad_batch_copy_image": (C.c_int, [C.c_void_p, C.c_int, fp]),
     "mrad_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
     "mrad_backward": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64, fp, fp, C.c_int]),

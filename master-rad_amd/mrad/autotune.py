@@ -1,0 +1,61 @@
+"""Autotuner of the wavefront form's launch parameters.
+
+Coordinate descent over queue capacity (paths in flight), waves per CU of
+the persistent kernels and the LDS split of the traversal stack (entries per
+lane in LDS), on a loaded batch: each candidate is applied with
+mrad_batch_configure (scenes stay resident), rendered once to warm and then
+timed.  Images do not depend on these parameters (every path runs the same
+arithmetic), so only time is compared.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+
+from .batch import BatchRenderer, WavefrontConfig
+
+GRID = {
+    "queue_capacity": [1 << 21, 1 << 22, 1 << 23, 1 << 24],
+    "waves_per_cu": [8, 12, 16, 24],
+    "lds_stack_entries": [4, 8],
+}
+
+
+def _time(batch: BatchRenderer, cfg: WavefrontConfig, repeats: int) -> float:
+    batch.configure(cfg)
+    batch.render()
+    best = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        batch.render()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune(batch: BatchRenderer, start: WavefrontConfig | None = None, grid: dict | None = None, passes: int = 1,
+         repeats: int = 1, log=None) -> tuple[WavefrontConfig, list[dict]]:
+    """Returns the fastest configuration found and the table of every trial."""
+    grid = grid or GRID
+    best = dataclasses.replace(start or batch.config, use_graph=True, time_kernels=False, collect_metrics=False)
+    seen: dict[tuple, float] = {}
+    table = []
+
+    def trial(cfg):
+        key = tuple(getattr(cfg, k) for k in grid)
+        if key not in seen:
+            seen[key] = _time(batch, cfg, repeats)
+            table.append({**{k: getattr(cfg, k) for k in grid}, "seconds": seen[key]})
+            if log:
+                log(f"autotune {table[-1]}")
+        return seen[key]
+
+    best_t = trial(best)
+    for _ in range(passes):
+        for name, values in grid.items():
+            for v in values:
+                cand = dataclasses.replace(best, **{name: v})
+                t = trial(cand)
+                if t < best_t:
+                    best, best_t = cand, t
+    batch.configure(best)
+    return best, table

@@ -26,13 +26,14 @@ class WavefrontConfig:
     use_graph: bool = True
     time_kernels: bool = False
     lds_stack_entries: int = 8
+    leaf_regroup: bool = False
     device: int = 0
 
     def native(self) -> N.WavefrontConfig:
         return N.WavefrontConfig(self.device, self.queue_capacity, self.waves_per_cu, self.bvh_stack_size,
                                  1 if self.linear_scan else 0, 1 if self.collect_metrics else 0,
                                  1 if self.use_graph else 0, 1 if self.time_kernels else 0,
-                                 self.lds_stack_entries)
+                                 self.lds_stack_entries, 1 if self.leaf_regroup else 0)
 
 
 def make_view(camera, width: int, height: int, samples: int, max_depth: int, frame_index: int, frame_seed: int,
@@ -80,6 +81,11 @@ class BatchRenderer:
     def render(self) -> dict:
         N.check(N.lib.mrad_batch_render(self._h))
         return self.stats()
+
+    def configure(self, config: WavefrontConfig) -> None:
+        """New launch parameters, scenes stay resident (mrad_batch_configure)."""
+        N.check(N.lib.mrad_batch_configure(self._h, C.byref(config.native())))
+        self.config = config
 
     def image(self, scene: int) -> np.ndarray:
         v = self.views[scene]

@@ -1,0 +1,88 @@
+"""Short, fixed workloads to run under rocprofv3 (kernel trace or counters).
+
+    python -m mrad.profile_run het        # 100k field, 1080p, 16 spp: megakernel frame + wavefront render
+    python -m mrad.profile_run batch      # the 64-scene batch, one wavefront render
+    python -m mrad.profile_run copy       # one device copy of 1 GiB (calibrates FETCH_SIZE / WRITE_SIZE)
+    python -m mrad.profile_run adjoint    # diff-10k forward + backward
+
+Run from master-rad_amd/ (or with it on sys.path).  Each prints one line per
+step so a profiled run shows progress.
+"""
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+
+
+def _het(spp: int):
+    import mrad
+    from mrad import configs
+    w = configs.het(100_000, spp=spp)
+    t = mrad.GpuRayTracer(w.width, w.height, w.spp, True)
+    t.set_russian_roulette(w.rr_start)
+    t.render_sample(None, w.scene, w.camera, w.max_depth, 0, 7)
+    print(f"megakernel frame {t.last_frame_stats()['kernel_nanos'] / 1e6:.1f} ms", flush=True)
+    t.close()
+    b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 23))
+    b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, 0, 7, w.rr_start)
+    st = b.render()
+    print(f"wavefront render {st['kernel_nanos'] / 1e6:.1f} ms, {st['steps']} steps", flush=True)
+    b.close()
+
+
+def _batch():
+    import mrad
+    from mrad import configs
+    ws = configs.batch(64, threads=16)
+    print("batch built", flush=True)
+    b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 23))
+    for w in ws:
+        b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, 0, w.frame_seed, w.rr_start)
+    st = b.render()
+    print(f"wavefront render {st['kernel_nanos'] / 1e6:.1f} ms, {st['steps']} steps", flush=True)
+    b.close()
+
+
+def _copy(nbytes: int = 1 << 30):
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"copy {nbytes} bytes in {dt * 1e3:.2f} ms ({nbytes * 2 / dt / 1e9:.0f} GB/s read+write)", flush=True)
+
+
+def _adjoint():
+    from mrad import configs
+    from mrad.adjoint import PathReplayRenderer, l2_loss
+    w = configs.diff(10_000)
+    r = PathReplayRenderer(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, frame_seed=0xD1FF)
+    img = r.forward()
+    _, g = l2_loss(img, img * np.float32(0.9))
+    r.backward(g)
+    print(f"backward {r.last_kernel_nanos() / 1e6:.1f} ms", flush=True)
+    r.close()
+
+
+def main(argv):
+    what = argv[1] if len(argv) > 1 else "het"
+    if what == "het":
+        _het(int(argv[2]) if len(argv) > 2 else 16)
+    elif what == "batch":
+        _batch()
+    elif what == "copy":
+        _copy()
+    elif what == "adjoint":
+        _adjoint()
+    else:
+        raise SystemExit(f"unknown workload {what}")
+
+
+if __name__ == "__main__":
+    main(sys.argv)

@@ -61,6 +61,9 @@ def _lib() -> C.CDLL:
     lib.or_adjoint.restype = C.c_int
     lib.or_adjoint.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                C.c_int, fp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.or_render_fixed_paths64.restype = C.c_int
+    lib.or_render_fixed_paths64.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                            C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.or64_closest.restype = C.c_int
     lib.or64_closest.argtypes = [C.POINTER(OrScene), fp, C.POINTER(C.c_double), ip, ip, C.POINTER(C.c_double)]
     return lib
@@ -177,6 +180,19 @@ def adjoint(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth
                         ga.ctypes.data_as(C.POINTER(C.c_double)), ge.ctypes.data_as(C.POINTER(C.c_double)))
     assert rc == 0
     return ga.reshape(-1, 3), ge.reshape(-1, 3)
+
+
+def render_fixed_paths64(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, frame_index: int,
+                         frame_seed: int, spp: int, materials64: np.ndarray) -> np.ndarray:
+    """The frame's FP32 paths with radiance re-evaluated in double from materials64 (n x 13)."""
+    cam = np.ascontiguousarray(camera, np.float32)
+    m = np.ascontiguousarray(materials64, np.float64).reshape(-1)
+    out = np.zeros(width * height * 3, np.float64)
+    rc = lib.or_render_fixed_paths64(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, max_depth, frame_index,
+                                     frame_seed & ((1 << 64) - 1), spp, m.ctypes.data_as(C.POINTER(C.c_double)),
+                                     out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    return out
 
 
 def closest64(scene: Scene, ray6):

@@ -872,6 +872,60 @@ int or_adjoint(const OrScene *s, const float *cam, int width, int height, int ma
 	return 0;
 }
 
+/* The frame's paths traced exactly as tracePath (FP32 geometry, the scene's
+ * own materials for every sampling decision) with throughput and radiance
+ * re-evaluated in double from override materials mat64 (13 doubles each).
+ * For Lambertian materials without roulette the path geometry does not depend
+ * on albedo or emission, so this image is the frame's estimator as an exact
+ * polynomial in those parameters: central differences of it are limited by
+ * double, not float, rounding. */
+int or_render_fixed_paths64(const OrScene *s, const float *cam, int width, int height, int maxDepth, int frameIndex,
+		uint64_t frameSeed, int spp, const double *mat64, double *out) {
+	if (!s || !cam || !mat64 || !out || spp < 1) return -1;
+	v3 pos = mk(cam[0], cam[1], cam[2]), fwd = mk(cam[3], cam[4], cam[5]);
+	v3 right = mk(cam[6], cam[7], cam[8]), up = mk(cam[9], cam[10], cam[11]);
+	for (int y = 0; y < height; y++)
+		for (int x = 0; x < width; x++) {
+			uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
+			double acc[3] = {0.0, 0.0, 0.0};
+			for (int so = 0; so < spp; so++) {
+				uint32_t rng = or_path_seed(frameSeed, frameIndex, pixel, so);
+				float jx = rand01(&rng) - 0.5f;
+				float jy = rand01(&rng) - 0.5f;
+				float aspect = (float) width / (float) height;
+				float u = ((((float) x + 0.5f) + jx) * (2.0f / (float) width) - 1.0f) * aspect;
+				float v = 1.0f - ((((float) y + 0.5f) + jy) * (2.0f / (float) height));
+				v3 o = pos, d = vadd(vadd(fwd, vmuls(right, u)), vmuls(up, v));
+				double T[3] = {1.0, 1.0, 1.0};
+				for (int depth = 0; depth < maxDepth; ++depth) {
+					Hit h;
+					if (!traceClosest(s, o, d, &h, NULL)) {
+						for (int c = 0; c < 3; c++) acc[c] += T[c] * (double) s->bg[c];
+						break;
+					}
+					const double *m = h.matIndex >= 0 ? mat64 + 13 * h.matIndex : NULL;
+					for (int c = 0; c < 3; c++) acc[c] += m ? T[c] * m[9 + c] : 0.0;
+					v3 nrm = h.normal, in = vneg(d);
+					float wD = luminance(h.m.diffuse), wR = luminance(h.m.reflective), wT = luminance(h.m.refractive);
+					float sum = wD + wR + wT;
+					if (sum <= 0.0f) break;
+					float xi = rand01(&rng) * sum;
+					v3 nd;
+					float p;
+					int lobe;
+					if (xi < wD) { p = wD / sum; nd = vadd(sampleUnitSphere(&rng), nrm); lobe = 0; }
+					else if (xi < wD + wR) { p = wR / sum; nd = reflect_dir(nrm, in); lobe = 1; }
+					else { p = wT / sum; v3 rd; refract_dir(in, nrm, h.m.ior, &rd); nd = rd; lobe = 2; }
+					for (int c = 0; c < 3; c++) T[c] *= (m ? m[3 * lobe + c] : 0.0) / (double) p;
+					o = vadd(h.pos, vmuls(nd, EPSILON));
+					d = nd;
+				}
+			}
+			for (int c = 0; c < 3; c++) out[3 * pixel + c] = acc[c] / spp;
+		}
+	return 0;
+}
+
 /* traceReplayKernel body, GpuKernelSources.java:1338-1376, with the per-ray
  * counters it computes kept per ray (perRay: n x 14 u32, may be NULL). */
 int or_trace_replay(const OrScene *s, const float *rays, int n, int *flags, float *dist, int *order,

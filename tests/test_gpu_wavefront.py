@@ -61,6 +61,42 @@ def test_batch_counters_equal_megakernel(require_gpu):
         assert st[k] == ref_stats[k], k
 
 
+@pytest.mark.parametrize("regroup,lds", [(True, 8), (False, 4)])
+def test_regroup_and_lds_split_bit_exact_with_counters(require_gpu, regroup, lds):
+    """Primitive-class regrouping of leaf tests and the 4-entry LDS split change neither images nor counters."""
+    cases = [(lambda: small_field(4000, 5, clustered=True), configs.field_camera, 80, 45, 3, 6, -1),
+             (gi_test_scene, gi_camera, 40, 40, 2, 12, -1),
+             (lambda: small_field(3000), configs.field_camera, 64, 36, 4, 8, 3)]
+    for k, (mk_scene, mk_cam, w, h, spp, depth, rr) in enumerate(cases):
+        sc, cam = mk_scene(), mk_cam()
+        seed = mrad.frame_seed(77 + k, 0)
+        ref, ref_stats = _mega(sc, cam, w, h, spp, depth, 0, seed, rr, metrics=True)
+        for metrics in (False, True):
+            cfg = mrad.WavefrontConfig(queue_capacity=5000, leaf_regroup=regroup, lds_stack_entries=lds,
+                                       collect_metrics=metrics)
+            with mrad.BatchRenderer(cfg) as b:
+                b.add_scene(sc, cam, w, h, spp, depth, 0, seed, rr)
+                st = b.render()
+                np.testing.assert_array_equal(b.image(0).view(np.uint32), ref.view(np.uint32))
+            if metrics:
+                for name in mrad.STAT_NAMES:
+                    assert st[name] == ref_stats[name], (k, name)
+
+
+def test_configure_keeps_scenes_and_images(require_gpu):
+    from mrad.autotune import tune
+    sc, cam = small_field(3000), configs.field_camera()
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=2048)) as b:
+        b.add_scene(sc, cam, 48, 27, 3, 6, 0, 11, 3)
+        b.render()
+        first = b.image(0)
+        best, table = tune(b, grid={"queue_capacity": [1024, 8192], "waves_per_cu": [4, 8],
+                                    "lds_stack_entries": [4, 8]})
+        assert len(table) >= 4
+        b.render()
+        np.testing.assert_array_equal(first, b.image(0))
+
+
 def test_batch_linear_scan_and_zero_depth(require_gpu):
     sc, cam = configs.room_scene(), configs.room_camera()
     with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1000, linear_scan=True)) as b:
