@@ -739,6 +739,24 @@ __device__ __forceinline__ bool roulette(int depth, int maxDepth, int rrStart, f
 	return true;
 }
 
+// One path segment after its closest hit has been resolved: emission,
+// scattering, throughput and roulette.  Returns false when the path ends here.
+__device__ __forceinline__ bool shadeResolved(const DevScene &S, const float *bgRGB, bool found, const HitDetail &h,
+                                              int depth, int maxDepth, int rrStart, f3 &o, f3 &d, f3 &T, f3 &L,
+                                              uint32_t &rng) {
+	if (!found) {
+		L = L + T * mk(bgRGB[0], bgRGB[1], bgRGB[2]);
+		return false;
+	}
+	MaterialF m = loadMaterial(S, h.matIndex);
+	L = L + T * m.emittance;
+	f3 w;
+	if (!scatter(m, h.normal, d, rng, w)) return false;
+	o = h.pos + d * kEps;
+	T = T * w;
+	return roulette(depth, maxDepth, rrStart, T, rng);
+}
+
 // One path segment after its closest-hit query: emission, scattering,
 // throughput and roulette.  Returns false when the path ends here.
 __device__ __forceinline__ bool shadeSegment(const DevScene &S, const SceneRange &R, bool found, const Best &best,

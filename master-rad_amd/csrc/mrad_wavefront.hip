@@ -47,7 +47,7 @@ struct WfPool {
 };
 
 // Device counter block.
-enum { kCount0 = 0, kCount1 = 1, kNextPath = 2, kWorkTrace = 3, kWorkShade = 4, kCounterWords = 16 };
+enum { kCount0 = 0, kCount1 = 1, kNextPath = 2, kWorkTrace = 3, kCounterWords = 16 };
 constexpr uint32_t kDead = 0xFFFFu;          // depth field of a path that never traces
 constexpr uint32_t kMissKind = 7u;
 
@@ -56,8 +56,8 @@ struct WfParams {
 	const WfScene *scenes;
 	int sceneCount;
 	WfPool pool[2];
-	float *hitT;
-	uint32_t *hitKI;        // kind << 29 | arena index
+	float4 *hitA;           // resolved closest hit: position, material index bits
+	float4 *hitB;           //                       normal, 1 if hit else 0
 	float *results;         // 3 floats per path
 	float *images;          // 3 floats per pixel, all scenes
 	uint32_t *counters;
@@ -130,12 +130,23 @@ __global__ void wfAdvanceGenerate(WfParams P, int cur) {
 	P.counters[kCount0 + cur] += n;
 	P.counters[kNextPath] += n;
 	P.counters[kWorkTrace] = 0;
-	P.counters[kWorkShade] = 0;
 }
 
 __global__ void wfAdvanceShade(WfParams P, int cur) {
 	if (threadIdx.x != 0 || blockIdx.x != 0) return;
 	P.counters[kCount0 + cur] = 0;
+}
+
+// The trace kernels resolve the winner (position, normal, material) with the
+// accepting routine's arithmetic, so shading never fetches primitive records.
+__device__ __forceinline__ void writeHit(const WfParams &P, uint32_t i, f3 o, f3 d, const Best &best, bool found) {
+	if (found) {
+		const HitDetail h = resolveHit(P.S, o, d, best.t, best.kind, best.index);
+		P.hitA[i] = make_float4(h.pos.x, h.pos.y, h.pos.z, __int_as_float(h.matIndex));
+		P.hitB[i] = make_float4(h.normal.x, h.normal.y, h.normal.z, 1.0f);
+	} else {
+		P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	}
 }
 
 // Next chunk of 64 items for this wave from a work counter.
@@ -145,9 +156,16 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 	return __shfl(base, 0, 64);
 }
 
+// Occupancy the trace kernels are compiled for (waves per SIMD; the register
+// budget is 512 / this).
+#ifndef MRAD_TRACE_WAVES_PER_EU
+#define MRAD_TRACE_WAVES_PER_EU 7
+#endif
+
 // Closest hit of every live path in pool `cur` (persistent).
 template <bool kMetrics, int kCap, int kShort>
-__global__ __launch_bounds__(kBlock) void wfTrace(WfParams P, int cur) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
+void wfTrace(WfParams P, int cur) {
 	__shared__ StackLdsT<kShort> stackLds;
 	const LdsStack lds = laneStack(stackLds);
 	const uint32_t count = P.counters[kCount0 + cur];
@@ -162,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void wfTrace(WfParams P, int cur) {
 		if (i < count) {
 			const uint4 meta = Q.meta[i];
 			if ((meta.z >> 16) == kDead) {
-				P.hitKI[i] = kMissKind << 29;
+				P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 				continue;
 			}
 			const float4 a = Q.od0[i], b = Q.od1[i];
@@ -171,73 +189,99 @@ __global__ __launch_bounds__(kBlock) void wfTrace(WfParams P, int cur) {
 			c.add(0, 1);
 			Best best;
 			const bool found = traceClosest<kMetrics, kCap, kShort>(P.S, R, o, d, best, c, lds);
-			P.hitT[i] = best.t;
-			P.hitKI[i] = found ? ((uint32_t) best.kind << 29) | (uint32_t) best.index : (kMissKind << 29);
+			writeHit(P, i, o, d, best, found);
 		}
 	}
 	if (P.stats) flushCounters<kMetrics>(c, P.stats);
 }
 
+// Shading of one pool entry; true when the path continues (its new state is
+// left in od0, od1, tl, meta), false when it finished (its radiance is written
+// to its result slot) or the entry is empty.
+__device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, uint32_t i, uint32_t count, float4 &od0,
+                                           float4 &od1, float4 &tl, uint4 &meta) {
+	if (i >= count) return false;
+	meta = Q.meta[i];
+	const uint32_t depth = meta.z >> 16;
+	if (depth == kDead) return false;
+	od0 = Q.od0[i];
+	od1 = Q.od1[i];
+	tl = Q.tl[i];
+	f3 o = mk(od0.x, od0.y, od0.z), d = mk(od0.w, od1.x, od1.y);
+	f3 T = mk(od1.z, od1.w, tl.x), L = mk(tl.y, tl.z, tl.w);
+	uint32_t rng = meta.x;
+	const WfScene &V = P.scenes[meta.z & 0xFFFFu];
+	const float4 ha = P.hitA[i], hb = P.hitB[i];
+	const bool found = hb.w != 0.0f;
+	HitDetail h;
+	h.pos = mk(ha.x, ha.y, ha.z);
+	h.matIndex = __float_as_int(ha.w);
+	h.normal = mk(hb.x, hb.y, hb.z);
+	const bool alive = shadeResolved(P.S, V.R.bg, found, h, (int) depth, V.maxDepth, V.rrStart, o, d, T, L, rng) &&
+	                   (int) depth + 1 < V.maxDepth;
+	if (alive) {
+		od0 = make_float4(o.x, o.y, o.z, d.x);
+		od1 = make_float4(d.y, d.z, T.x, T.y);
+		tl = make_float4(T.z, L.x, L.y, L.z);
+		meta.x = rng;
+		meta.z = (meta.z & 0xFFFFu) | ((depth + 1) << 16);
+	} else {
+		float *r = P.results + 3 * (size_t) meta.y;
+		r[0] = L.x;
+		r[1] = L.y;
+		r[2] = L.z;
+	}
+	return alive;
+}
+
 // Shading and compaction of every live path of pool `cur` into pool cur ^ 1.
+// Shading costs the same for every path, so blocks take fixed tiles of
+// kShadeTile entries (kShadePer per thread) in turn; the survivors of a tile
+// keep their order and take one atomic per tile on the destination count
+// (per-wave atomics on one address serialise and cost more than the shading).
+constexpr int kShadePer = 4;
+constexpr uint32_t kShadeTile = kBlock * kShadePer;
+
 __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
+	__shared__ uint32_t part[kShadePer * kWaves];
+	__shared__ uint32_t tileBase;
 	const uint32_t count = P.counters[kCount0 + cur];
 	const WfPool Q = P.pool[cur], D = P.pool[cur ^ 1];
-	const int lane = threadIdx.x & 63;
-	for (;;) {
-		const uint32_t base = fetchChunk(P.counters + kWorkShade);
-		if (base >= count) break;
-		const uint32_t i = base + lane;
-		bool alive = false;
-		float4 od0, od1, tl;
-		uint4 meta;
-		if (i < count) {
-			meta = Q.meta[i];
-			const uint32_t depth = meta.z >> 16;
-			if (depth != kDead) {
-				od0 = Q.od0[i];
-				od1 = Q.od1[i];
-				tl = Q.tl[i];
-				f3 o = mk(od0.x, od0.y, od0.z), d = mk(od0.w, od1.x, od1.y);
-				f3 T = mk(od1.z, od1.w, tl.x), L = mk(tl.y, tl.z, tl.w);
-				uint32_t rng = meta.x;
-				const WfScene &V = P.scenes[meta.z & 0xFFFFu];
-				const uint32_t ki = P.hitKI[i];
-				const int kind = (int) (ki >> 29);
-				Best best;
-				best.t = P.hitT[i];
-				best.kind = kind == (int) kMissKind ? -1 : kind;
-				best.index = (int) (ki & 0x1FFFFFFFu);
-				best.rank = 0;
-				const bool found = kind != (int) kMissKind;
-				alive = shadeSegment(P.S, V.R, found, best, (int) depth, V.maxDepth, V.rrStart, o, d, T, L, rng) &&
-				        (int) depth + 1 < V.maxDepth;
-				if (alive) {
-					od0 = make_float4(o.x, o.y, o.z, d.x);
-					od1 = make_float4(d.y, d.z, T.x, T.y);
-					tl = make_float4(T.z, L.x, L.y, L.z);
-					meta.x = rng;
-					meta.z = (meta.z & 0xFFFFu) | ((depth + 1) << 16);
-				} else {
-					float *r = P.results + 3 * (size_t) meta.y;
-					r[0] = L.x;
-					r[1] = L.y;
-					r[2] = L.z;
-				}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	for (uint32_t tile = blockIdx.x * kShadeTile; tile < count; tile += gridDim.x * kShadeTile) {
+		float4 od0[kShadePer], od1[kShadePer], tl[kShadePer];
+		uint4 meta[kShadePer];
+		bool alive[kShadePer];
+		unsigned long long mask[kShadePer];
+#pragma unroll
+		for (int k = 0; k < kShadePer; k++) {
+			alive[k] = shadeEntry(P, Q, tile + k * kBlock + threadIdx.x, count, od0[k], od1[k], tl[k], meta[k]);
+			mask[k] = __ballot(alive[k]);
+			if (lane == 0) part[k * kWaves + wave] = (uint32_t) __popcll(mask[k]);
+		}
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			uint32_t sum = 0;
+			for (int j = 0; j < kShadePer * kWaves; j++) {
+				const uint32_t n = part[j];
+				part[j] = sum;
+				sum += n;
+			}
+			tileBase = sum ? atomicAdd(P.counters + kCount0 + (cur ^ 1), sum) : 0u;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < kShadePer; k++) {
+			if (alive[k]) {
+				const uint32_t slot = tileBase + part[k * kWaves + wave] + (uint32_t) __popcll(mask[k] & below);
+				D.od0[slot] = od0[k];
+				D.od1[slot] = od1[k];
+				D.tl[slot] = tl[k];
+				D.meta[slot] = meta[k];
 			}
 		}
-		// wave64 compaction: one atomic per wave, prefix of the ballot per lane
-		const unsigned long long mask = __ballot(alive);
-		const uint32_t n = (uint32_t) __popcll(mask);
-		uint32_t dst = 0;
-		if (lane == 0 && n) dst = atomicAdd(P.counters + kCount0 + (cur ^ 1), n);
-		dst = __shfl(dst, 0, 64);
-		if (alive) {
-			const uint32_t slot = dst + (uint32_t) __popcll(mask & ((1ull << lane) - 1ull));
-			D.od0[slot] = od0;
-			D.od1[slot] = od1;
-			D.tl[slot] = tl;
-			D.meta[slot] = meta;
-		}
+		__syncthreads();
 	}
 }
 
@@ -327,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 		if (valid) {
 			meta = Q.meta[i];
 			if ((meta.z >> 16) == kDead) {
-				P.hitKI[i] = kMissKind << 29;
+				P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 				valid = false;
 			}
 		}
@@ -507,8 +551,7 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 			c.add(2, (uint32_t) R.planeCount);
 			c.add(5, (uint32_t) R.planeCount);
 			tracePlanes(P.S, R, o, d, best);
-			P.hitT[i] = best.t;
-			P.hitKI[i] = best.kind >= 0 ? ((uint32_t) best.kind << 29) | (uint32_t) best.index : (kMissKind << 29);
+			writeHit(P, i, o, d, best, best.kind >= 0);
 		}
 	}
 	if (P.stats) flushCounters<kMetrics>(c, P.stats);
@@ -537,11 +580,12 @@ struct mrad_batch {
 	hipStream_t stream = nullptr;
 	hipEvent_t evStart = nullptr, evEnd = nullptr;
 	int cuCount = 0;
+	int shadeBlocksPerCu = 1;           // resident shade blocks per CU
 	HostArena host;
 	std::vector<WfScene> scenes;
 	bool dirty = true;
 	DevArena arena;
-	DevBuf sceneTable, pool[2][4], hitT, hitKI, results, images, counters, stats;
+	DevBuf sceneTable, pool[2][4], hitA, hitB, results, images, counters, stats;
 	hipGraphExec_t graph = nullptr;
 	std::vector<hipEvent_t> stepEvents;
 	uint32_t totalPaths = 0, totalPixels = 0;
@@ -571,8 +615,8 @@ int prepare(mrad_batch *b) {
 	const uint32_t cap = (uint32_t) b->cfg.queue_capacity;
 	for (int q = 0; q < 2; q++)
 		for (int k = 0; k < 4; k++) MRAD_TRY(ensure(b->pool[q][k], (size_t) cap * 16));
-	MRAD_TRY(ensure(b->hitT, (size_t) cap * 4));
-	MRAD_TRY(ensure(b->hitKI, (size_t) cap * 4));
+	MRAD_TRY(ensure(b->hitA, (size_t) cap * 16));
+	MRAD_TRY(ensure(b->hitB, (size_t) cap * 16));
 	MRAD_TRY(ensure(b->results, (size_t) b->totalPaths * 12));
 	MRAD_TRY(ensure(b->images, (size_t) b->totalPixels * 12));
 	MRAD_TRY(ensure(b->counters, kCounterWords * 4));
@@ -593,8 +637,8 @@ WfParams params(mrad_batch *b) {
 		P.pool[q].tl = b->pool[q][2].as<float4>();
 		P.pool[q].meta = b->pool[q][3].as<uint4>();
 	}
-	P.hitT = b->hitT.as<float>();
-	P.hitKI = b->hitKI.as<uint32_t>();
+	P.hitA = b->hitA.as<float4>();
+	P.hitB = b->hitB.as<float4>();
 	P.results = b->results.as<float>();
 	P.images = b->images.as<float>();
 	P.counters = b->counters.as<uint32_t>();
@@ -609,6 +653,9 @@ WfParams params(mrad_batch *b) {
 // `ev` (4 events) the kernel boundaries are recorded for per-kernel timing.
 int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = nullptr) {
 	const int persistentBlocks = std::max(1, b->cuCount * b->cfg.waves_per_cu / kWaves);
+	// shading takes fixed tiles: one resident wave of blocks, whatever the
+	// trace kernel's setting
+	const int shadeBlocks = std::max(1, b->cuCount * b->shadeBlocksPerCu);
 	const int genBlocks = std::max(1, std::min(8192, (int) ((b->cfg.queue_capacity + kBlock - 1) / kBlock)));
 	if (ev) MRAD_HIP(hipEventRecord(ev[0], b->stream));
 	hipLaunchKernelGGL(wfGenerate, dim3(genBlocks), dim3(kBlock), 0, b->stream, P, cur);
@@ -620,7 +667,7 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 	                          : traceFor<false>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries, regroup);
 	hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
-	hipLaunchKernelGGL(wfShade, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	hipLaunchKernelGGL(wfShade, dim3(shadeBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	hipLaunchKernelGGL(wfAdvanceShade, dim3(1), dim3(64), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[3], b->stream));
 	MRAD_HIP(hipGetLastError());
@@ -671,7 +718,7 @@ int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out) {
 	*out = nullptr;
 	mrad_wavefront_config c = *config;
 	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
-	if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
 	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
 	if (c.lds_stack_entries == 0) c.lds_stack_entries = 8;
 	if (c.lds_stack_entries != 4 && c.lds_stack_entries != 8)
@@ -697,7 +744,11 @@ int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out) {
 	           (e = hipDeviceGetAttribute(&b->cuCount, hipDeviceAttributeMultiprocessorCount, c.device)) != hipSuccess ||
 	           (e = hipHostMalloc((void **) &b->hostCounters, kCounterWords * 4, hipHostMallocDefault)) != hipSuccess) {
 		b->unavailableReason = hipGetErrorString(e);
+	} else if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->shadeBlocksPerCu, wfShade, kBlock, 0)) !=
+	           hipSuccess) {
+		b->unavailableReason = hipGetErrorString(e);
 	} else {
+		b->shadeBlocksPerCu = std::max(1, b->shadeBlocksPerCu);
 		b->available = true;
 	}
 	*out = b;
@@ -715,7 +766,7 @@ void mrad_batch_destroy(mrad_batch *b) {
 			b->arena.release();
 			for (int q = 0; q < 2; q++)
 				for (int k = 0; k < 4; k++) b->pool[q][k].release();
-			for (DevBuf *x : {&b->sceneTable, &b->hitT, &b->hitKI, &b->results, &b->images, &b->counters, &b->stats})
+			for (DevBuf *x : {&b->sceneTable, &b->hitA, &b->hitB, &b->results, &b->images, &b->counters, &b->stats})
 				x->release();
 			if (b->hostCounters) (void) hipHostFree(b->hostCounters);
 			for (hipEvent_t e : b->stepEvents) (void) hipEventDestroy(e);
@@ -830,7 +881,7 @@ int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
 	if (!config) return set_error(MRAD_ERR_INVALID_ARGUMENT, "config is required");
 	mrad_wavefront_config c = *config;
 	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
-	if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
 	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
 	if (c.lds_stack_entries == 0) c.lds_stack_entries = 8;
 	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28) || c.waves_per_cu < 1 || c.waves_per_cu > 32 ||
@@ -847,8 +898,8 @@ int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
 	if (grow && !b->dirty) {
 		for (int q = 0; q < 2; q++)
 			for (int k = 0; k < 4; k++) MRAD_TRY(ensure(b->pool[q][k], (size_t) c.queue_capacity * 16));
-		MRAD_TRY(ensure(b->hitT, (size_t) c.queue_capacity * 4));
-		MRAD_TRY(ensure(b->hitKI, (size_t) c.queue_capacity * 4));
+		MRAD_TRY(ensure(b->hitA, (size_t) c.queue_capacity * 16));
+		MRAD_TRY(ensure(b->hitB, (size_t) c.queue_capacity * 16));
 	}
 	return MRAD_OK;
 }
