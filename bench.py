@@ -17,7 +17,10 @@ every scene resident in HBM and every image left in HBM.  Reported beside it:
     primitives, 512^2, 32 spp, 4 bounces);
   * the FP64 CPU renderer timed on a bounded sample on this host.
 
---workload het runs the single 100k-primitive 1080p, 64 spp megakernel frame.
+--workload het (100k primitives, 1080p, 64 spp, 8 bounces) and --workload
+clustered (1M clustered primitives, half affine, 1080p, 16 spp, 6 bounces) time
+one full-size frame in the megakernel, the wavefront and the regrouped
+wavefront forms.
 """
 from __future__ import annotations
 
@@ -85,9 +88,56 @@ def cpu_baseline(w: configs.Workload, seconds: float = 12.0) -> dict:
     O.render64(osc, cam, sw, sh, w.max_depth, spp, 2, rr_start=w.rr_start, threads=threads)
     dt = time.perf_counter() - t0
     paths = sw * sh * spp
+    # one thread on a 1/4-scale frame of the same camera
+    qw, qh = max(8, w.width // 4), max(8, w.height // 4)
+    t0 = time.perf_counter()
+    O.render64(osc, cam, qw, qh, w.max_depth, 1, 3, rr_start=w.rr_start, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"{w.name} ({w.scene.primitive_count()} primitives) camera at {sw}x{sh}, {spp} spp, "
-                      f"depth {w.max_depth}: {paths} paths in {dt:.1f} s, FP64 CPU renderer"}
+                      f"depth {w.max_depth}: {paths} paths in {dt:.1f} s, FP64 CPU renderer",
+            "single_thread_value": qw * qh / dt1 / 1e6,
+            "single_thread_sample": f"{qw}x{qh}, 1 spp: {qw * qh} paths in {dt1:.2f} s",
+            "host": host_cpu()}
+
+
+def host_cpu() -> str:
+    name = "unknown CPU"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs visible"
+
+
+def relmse_check(ws, ref_spp: int = 1024) -> dict:
+    """relMSE = mean (I - R)^2 / (R^2 + 1e-2) of the GPU image of the batch's
+    smallest 256^2 scene against a converged FP64 CPU reference R of the same
+    scene and camera (independent random numbers).  The wavefront and
+    megakernel images are bit-identical, so this is the relMSE of both forms;
+    `expected` is the Monte-Carlo expectation mean(var_I / (R^2 + 1e-2)) from
+    the per-pixel variance of the reference run."""
+    from oracle import oracle as O
+    w = min((w for w in ws if w.width == 256), key=lambda w: w.scene.primitive_count())
+    t = mrad.GpuRayTracer(w.width, w.height, w.spp, True)
+    t.set_russian_roulette(w.rr_start)
+    img = np.zeros(w.width * w.height * 3, np.float32)
+    t.render_sample(img, w.scene, w.camera, w.max_depth, 0, w.frame_seed)
+    t.close()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    ref, var = O.render64(O.Scene(w.scene.wire()), w.camera.packed(), w.width, w.height, w.max_depth, ref_spp, 0x5EED,
+                          rr_start=w.rr_start, threads=threads, variance=True)
+    dt = time.perf_counter() - t0
+    ref = ref.reshape(-1)
+    den = ref ** 2 + 1e-2
+    rel = float(np.mean((img.astype(np.float64) - ref) ** 2 / den))
+    expected = float(np.mean(var.reshape(-1) * (1.0 / w.spp + 1.0 / ref_spp) / den))
+    return {"scene": f"{w.name} ({w.scene.primitive_count()} primitives) {w.width}x{w.height}", "gpu_spp": w.spp,
+            "reference_spp": ref_spp, "relmse": rel, "expected_relmse": expected, "reference_seconds": dt}
 
 
 def make_batch(ws, **cfg) -> mrad.BatchRenderer:
@@ -210,13 +260,15 @@ def bench_adjoint() -> dict:
     return {"workload": "diff-10k 512x512 32 spp 4 bounces", "forward_ms": f, "backward_ms": bk, "ratio": bk / f}
 
 
-def bench_het(args) -> dict:
-    w = configs.het(100_000)
+def bench_single(args, w: configs.Workload) -> dict:
+    """One full-size frame: the plain megakernel (the line's value) beside the
+    wavefront form and the wavefront form with leaf regrouping by primitive
+    class, all three on the same frame (bit-identical images)."""
     stats = w.scene.bvh_stats()
+    seed = 0x4D524144
     t = mrad.GpuRayTracer(w.width, w.height, w.spp, True)
     t.set_russian_roulette(w.rr_start)
     cam = w.camera.packed()
-    seed = 0x4D524144
     for i in range(args.warmup):
         t.render_sample(None, w.scene, w.camera, w.max_depth, i, mrad.frame_seed(seed, i))
     kernel_ns = []
@@ -226,6 +278,8 @@ def bench_het(args) -> dict:
         t.render_sample(None, w.scene, cam, w.max_depth, fi, mrad.frame_seed(seed, fi))
         kernel_ns.append(t.last_frame_stats()["kernel_nanos"])
     wall = time.perf_counter() - t0
+    mk_img = np.zeros(w.width * w.height * 3, np.float32)
+    t.copy_frame(mk_img)
     t.close()
     tm = mrad.GpuRayTracer(w.width, w.height, w.spp, True, collect_metrics=True)
     tm.set_russian_roulette(w.rr_start)
@@ -237,6 +291,31 @@ def bench_het(args) -> dict:
         totals = st if totals is None else {k: (totals[k] + st[k]) if k != "max_stack_size"
                                             else max(totals[k], st[k]) for k in st}
     tm.close()
+    log(f"megakernel {wall / args.steps * 1e3:.0f} ms/frame")
+    # the last frame again in the wavefront form, plain and regrouped
+    last = args.warmup + args.steps - 1
+    forms = {}
+    for name, regroup in (("wavefront", 0), ("wavefront_regroup", 1)):
+        b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=args.queue_capacity,
+                                                    waves_per_cu=args.waves_per_cu, leaf_regroup=regroup))
+        b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, last, mrad.frame_seed(seed, last),
+                    w.rr_start)
+        b.render()
+        t1 = time.perf_counter()
+        st = b.render()
+        dt = time.perf_counter() - t1
+        same = bool(np.array_equal(b.image(0).view(np.uint32), mk_img.view(np.uint32)))
+        b.close()
+        bt = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=args.queue_capacity,
+                                                     waves_per_cu=args.waves_per_cu, leaf_regroup=regroup,
+                                                     time_kernels=True))
+        bt.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, last, mrad.frame_seed(seed, last),
+                     w.rr_start)
+        stt = bt.render()
+        bt.close()
+        forms[name] = {"ms": dt * 1e3, "mpaths_per_s": w.paths / dt / 1e6, "trace_ms": stt["trace_nanos"] / 1e6,
+                       "image_bit_identical_to_megakernel": same}
+        log(f"{name} {dt * 1e3:.0f} ms/frame (trace {stt['trace_nanos'] / 1e6:.0f} ms), bit-identical: {same}")
     paths = w.paths * args.steps
     kernel_s = sum(kernel_ns) / 1e9
     achieved = (algorithmic_bytes(totals) + 12.0 * w.width * w.height * args.steps) / kernel_s / 1e9
@@ -244,12 +323,13 @@ def bench_het(args) -> dict:
         "metric": METRIC, "value": paths / wall / 1e6, "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": "het-100k megakernel", "primitives": 100_000, "width": w.width, "height": w.height,
-                   "spp": w.spp, "max_depth": w.max_depth, "russian_roulette_after": w.rr_start,
-                   "bvh": f"weighted SAH leaf 8, {stats.node_count} nodes"},
+        "config": {"workload": f"{w.name} megakernel", "primitives": w.scene.primitive_count(), "width": w.width,
+                   "height": w.height, "spp": w.spp, "max_depth": w.max_depth, "russian_roulette_after": w.rr_start,
+                   "bvh": f"{w.scene.config.mode.split('_')[0].lower()} SAH leaf {w.scene.config.leaf_size}, {stats.node_count} nodes"},
         "mrays_per_s": totals["rays"] / wall / 1e6, "segments_per_path": totals["rays"] / paths,
+        "forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("het-100k"), "kernel": "renderKernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(w.name), "kernel": "renderKernel",
                      "kernel_ms_per_step": kernel_s / args.steps * 1e3},
         "counters_per_step": {k: totals[k] / args.steps for k in mrad.STAT_NAMES},
     }, [w]
@@ -260,7 +340,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "het"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "het", "clustered"], default="batch")
     ap.add_argument("--queue-capacity", type=int, default=1 << 23)
     ap.add_argument("--waves-per-cu", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -269,12 +349,19 @@ def main():
         print("bench.py measures one MI355X: every frame and batch renders on one GPU and this build has no "
               "multi-GPU form; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res, ws = bench_batch(args) if args.workload == "batch" else bench_het(args)
+    if args.workload == "batch":
+        res, ws = bench_batch(args)
+        # the largest 1080p scene of the batch
+        sample = max((w for w in ws if w.width == 1920), key=lambda w: w.scene.primitive_count())
+    else:
+        res, ws = bench_single(args, configs.het(100_000) if args.workload == "het" else configs.clustered())
+        sample = ws[0]
     if args.no_cpu_baseline:
         res["cpu_baseline"] = None
     else:
-        sample = configs.het(100_000) if args.workload == "het" else ws[max(range(len(ws)), key=lambda i: ws[i].scene.primitive_count() if ws[i].width == 1920 else 0)]
         res["cpu_baseline"] = cpu_baseline(sample)
+        if args.workload == "batch":
+            res["cpu_baseline"]["relmse"] = relmse_check(ws)
     print(json.dumps(res), flush=True)
 
 

@@ -161,6 +161,7 @@ struct SceneRange {
 	int planeBase, planeCount;
 	int useBvh;                 // BVH traversal (1) or linear scan (0)
 	int stackLimit;             // BVH_STACK_SIZE semantics
+	int fastSlabs;              // every node and box bound is 0 or of magnitude in [2^-35, 2^58]
 	float bg[3];
 };
 
@@ -181,24 +182,94 @@ __device__ __forceinline__ MaterialF loadMaterial(const DevScene &S, int index) 
 	return m;
 }
 
+// ------------------------------------------------------------ ray constants
+//
+// Every slab and sphere test of a closest-hit query divides by the same few
+// values: a direction component, or dot(d, d).  The quotients must be the
+// correctly rounded ones the reference computes, so instead of one IEEE
+// division per slab (about eleven instructions with a quarter-rate reciprocal)
+// a query computes the correctly rounded reciprocal y = RN(1/b) of each
+// divisor once, and each quotient a/b as
+//     q = RN(a y),  r = a - b q (exact, one fma),  RN(q + r y) (one fma),
+// which equals RN(a/b) when y = RN(1/b) and nothing under- or overflows
+// (P. Markstein, "Computation of elementary functions on the IBM RISC
+// System/6000 processor", IBM J. Res. Dev. 34(1), 1990, Theorem 1).  The
+// range kept is |a|, |b| in [2^-60, 2^60], where q, r and the result stay
+// normal; a = 0 returns q = RN(0 y), which is 0/b with its sign.
+//
+// A slab numerator is a bound minus an origin coordinate.  When both are 0 or
+// of magnitude in [2^-35, 2^58], their difference is 0 or a multiple of 2^-58
+// below 2^59, so it lies in range without a test: the scene records whether
+// its node and box bounds qualify, the ray whether its origin and direction
+// do.  Rays that do not qualify (and a sphere or affine numerator, which is
+// computed, when out of range) take the IEEE division.
+struct Ray {
+	f3 o, d;
+	f3 inv;        // RN(1 / d.c), 0 where d.c == 0
+	float dd;      // dot(d, d)
+	float invDD;   // RN(1 / dd)
+	bool fast;     // slab quotients of this ray may use the reciprocals
+};
+__device__ __forceinline__ bool magnitudeIn(float v, float lo, float hi) {
+	const float a = fabsf(v);
+	return a >= lo && a <= hi;
+}
+__device__ __forceinline__ Ray makeRay(f3 o, f3 d, bool sceneFast) {
+	Ray r;
+	r.o = o;
+	r.d = d;
+	r.inv = mk(d.x != 0.0f ? 1.0f / d.x : 0.0f, d.y != 0.0f ? 1.0f / d.y : 0.0f, d.z != 0.0f ? 1.0f / d.z : 0.0f);
+	r.dd = dot(d, d);
+	r.invDD = 1.0f / r.dd;
+	auto dirOk = [](float v) { return v == 0.0f || magnitudeIn(v, 0x1p-60f, 0x1p60f); };
+	auto origOk = [](float v) { return v == 0.0f || magnitudeIn(v, 0x1p-35f, 0x1p58f); };
+	r.fast = sceneFast && dirOk(d.x) && dirOk(d.y) && dirOk(d.z) && magnitudeIn(r.dd, 0x1p-60f, 0x1p60f) &&
+	         origOk(o.x) && origOk(o.y) && origOk(o.z);
+	return r;
+}
+// RN(num / den) from inv = RN(1 / den); num == 0 or |num| in [2^-60, 2^60].
+__device__ __forceinline__ float exactQuotient(float num, float den, float inv) {
+	const float q = num * inv;
+	const float c = fmaf(fmaf(-q, den, num), inv, q);
+	return num == 0.0f ? q : c;
+}
+__device__ __forceinline__ bool quotientOk(float num) { return num == 0.0f || magnitudeIn(num, 0x1p-60f, 0x1p60f); }
+// num / den of a computed numerator: the reciprocal form when the ray allows
+// it and num is in range, the IEEE division otherwise.
+__device__ __forceinline__ float quotient(float num, float den, float inv, bool fast) {
+	if (fast && quotientOk(num)) return exactQuotient(num, den, inv);
+	return num / den;
+}
+
 // ------------------------------------------------------------ intersection
 // Each test returns the reference's "found" predicate against `limit`
 // (closestT in the reference) and writes the candidate distance.
 
 // traceSphere body (GpuKernelSources.java:343-381).
 struct SphereSolve { float t, radialOffset; f3 closest; };
-__device__ __forceinline__ bool sphereSolve(float4 s, f3 o, f3 d, SphereSolve &r) {
-	f3 oc = o - mk(s.x, s.y, s.z);
-	float a = dot(d, d);
+// kFast: quotients by dd's reciprocal; `ok` is cleared if a numerator is out
+// of range (the result is then discarded).
+template <bool kFast>
+__device__ __forceinline__ bool sphereSolveT(float4 s, const Ray &ray, SphereSolve &r, bool &ok) {
+	auto div = [&](float num) {
+		if constexpr (kFast) {
+			ok = ok && quotientOk(num);
+			return exactQuotient(num, ray.dd, ray.invDD);
+		} else {
+			return num / ray.dd;
+		}
+	};
+	const f3 d = ray.d;
+	f3 oc = ray.o - mk(s.x, s.y, s.z);
 	float b = dot(oc, d);
-	float centerT = -b / a;
+	float centerT = div(-b);
 	f3 cl = mk(fmaf(d.x, centerT, oc.x), fmaf(d.y, centerT, oc.y), fmaf(d.z, centerT, oc.z));
-	float corr = -dot(cl, d) / a;
+	float corr = div(-dot(cl, d));
 	centerT += corr;
 	cl = mk(fmaf(d.x, corr, cl.x), fmaf(d.y, corr, cl.y), fmaf(d.z, corr, cl.z));
 	float rs = fmaf(s.w, s.w, -dot(cl, cl));
 	if (rs <= 0.0f) return false;
-	float halfSpan = sqrtf(rs / a);
+	float halfSpan = sqrtf(div(rs));
 	float t1 = centerT - halfSpan, t2 = centerT + halfSpan;
 	if (t1 > kEps) { r.t = t1; r.radialOffset = -halfSpan; }
 	else if (t2 > kEps) { r.t = t2; r.radialOffset = halfSpan; }
@@ -206,9 +277,21 @@ __device__ __forceinline__ bool sphereSolve(float4 s, f3 o, f3 d, SphereSolve &r
 	r.closest = cl;
 	return true;
 }
-__device__ __forceinline__ bool testSphere(float4 s, f3 o, f3 d, float limit, float &t) {
+__device__ __forceinline__ bool sphereSolve(float4 s, const Ray &ray, SphereSolve &r) {
+	bool ok = true;
+	if (ray.fast) {
+		SphereSolve f;
+		const bool hit = sphereSolveT<true>(s, ray, f, ok);
+		if (ok) {
+			r = f;
+			return hit;
+		}
+	}
+	return sphereSolveT<false>(s, ray, r, ok);
+}
+__device__ __forceinline__ bool testSphere(float4 s, const Ray &ray, float limit, float &t) {
 	SphereSolve r;
-	if (!sphereSolve(s, o, d, r)) return false;
+	if (!sphereSolve(s, ray, r)) return false;
 	t = r.t;
 	return r.t < limit;
 }
@@ -222,6 +305,18 @@ __device__ __forceinline__ bool slab(float oc, float dc, float lo, float hi, flo
 		return true;
 	}
 	float t1 = (lo - oc) / dc, t2 = (hi - oc) / dc;
+	tmin = fminf(t1, t2); tmax = fmaxf(t1, t2);
+	return true;
+}
+
+// The same slab of a world-space box by the ray's reciprocal of dc.
+__device__ __forceinline__ bool slabFast(float oc, float dc, float inv, float lo, float hi, float &tmin, float &tmax) {
+	if (dc == 0.0f) {
+		if (oc < lo || oc > hi) return false;
+		tmin = -INFINITY; tmax = INFINITY;
+		return true;
+	}
+	float t1 = exactQuotient(lo - oc, dc, inv), t2 = exactQuotient(hi - oc, dc, inv);
 	tmin = fminf(t1, t2); tmax = fmaxf(t1, t2);
 	return true;
 }
@@ -253,16 +348,23 @@ __device__ __forceinline__ f3 axisNormal(int axis, float sign) {
 }
 
 // traceBox body (GpuKernelSources.java:594-684).
-__device__ __forceinline__ bool boxSolve(float4 b0, float4 b1, f3 o, f3 d, BoxSolve &r) {
+__device__ __forceinline__ bool boxSolve(float4 b0, float4 b1, const Ray &ray, BoxSolve &r) {
 	float x0, x1, y0, y1, z0, z1;
-	if (!slab(o.x, d.x, b0.x, b0.w, x0, x1)) return false;
-	if (!slab(o.y, d.y, b0.y, b1.x, y0, y1)) return false;
-	if (!slab(o.z, d.z, b0.z, b1.y, z0, z1)) return false;
+	const f3 o = ray.o, d = ray.d;
+	if (ray.fast) {
+		if (!slabFast(o.x, d.x, ray.inv.x, b0.x, b0.w, x0, x1)) return false;
+		if (!slabFast(o.y, d.y, ray.inv.y, b0.y, b1.x, y0, y1)) return false;
+		if (!slabFast(o.z, d.z, ray.inv.z, b0.z, b1.y, z0, z1)) return false;
+	} else {
+		if (!slab(o.x, d.x, b0.x, b0.w, x0, x1)) return false;
+		if (!slab(o.y, d.y, b0.y, b1.x, y0, y1)) return false;
+		if (!slab(o.z, d.z, b0.z, b1.y, z0, z1)) return false;
+	}
 	return boxSelect(x0, x1, y0, y1, z0, z1, kEps, d, r);
 }
-__device__ __forceinline__ bool testBox(float4 b0, float4 b1, f3 o, f3 d, float limit, float &t) {
+__device__ __forceinline__ bool testBox(float4 b0, float4 b1, const Ray &ray, float limit, float &t) {
 	BoxSolve r;
-	if (!boxSolve(b0, b1, o, d, r)) return false;
+	if (!boxSolve(b0, b1, ray, r)) return false;
 	t = r.t;
 	return !(r.t >= limit);
 }
@@ -270,10 +372,11 @@ __device__ __forceinline__ bool testBox(float4 b0, float4 b1, f3 o, f3 d, float 
 // affineRayShift / applyAffine* (GpuKernelSources.java:276-301); rows hold
 // the inverse linear map with the world centre in .w.
 struct AffineLocal { float shift; f3 lo, ld; };
-__device__ __forceinline__ void affineLocal(float4 r0, float4 r1, float4 r2, f3 o, f3 d, AffineLocal &a) {
+__device__ __forceinline__ void affineLocal(float4 r0, float4 r1, float4 r2, const Ray &ray, AffineLocal &a) {
+	const f3 o = ray.o, d = ray.d;
 	f3 c = mk(r0.w, r1.w, r2.w);
-	float den = dot(d, d);
-	a.shift = den > 0.0f ? fmaxf(0.0f, dot(c - o, d) / den) : 0.0f;
+	float den = ray.dd;
+	a.shift = den > 0.0f ? fmaxf(0.0f, quotient(dot(c - o, d), den, ray.invDD, ray.fast)) : 0.0f;
 	f3 rel = mk(fmaf(d.x, a.shift, o.x - r0.w), fmaf(d.y, a.shift, o.y - r1.w), fmaf(d.z, a.shift, o.z - r2.w));
 	a.lo = mk(r0.x * rel.x + r0.y * rel.y + r0.z * rel.z,
 	          r1.x * rel.x + r1.y * rel.y + r1.z * rel.z,
@@ -326,9 +429,9 @@ __device__ __forceinline__ bool affineSphereSolve(const AffineLocal &a, float &h
 	worldT = a.shift + hitT;
 	return true;
 }
-__device__ __forceinline__ bool testAffineSphere(const float4 *rec, f3 o, f3 d, float limit, float &t) {
+__device__ __forceinline__ bool testAffineSphere(const float4 *rec, const Ray &ray, float limit, float &t) {
 	AffineLocal a;
-	affineLocal(rec[0], rec[1], rec[2], o, d, a);
+	affineLocal(rec[0], rec[1], rec[2], ray, a);
 	float hitT, worldT;
 	if (!affineSphereSolve(a, hitT, worldT)) return false;
 	t = worldT;
@@ -345,9 +448,9 @@ __device__ __forceinline__ bool affineBoxSolve(const AffineLocal &a, BoxSolve &r
 	worldT = a.shift + r.t;
 	return true;
 }
-__device__ __forceinline__ bool testAffineBox(const float4 *rec, f3 o, f3 d, float limit, float &t) {
+__device__ __forceinline__ bool testAffineBox(const float4 *rec, const Ray &ray, float limit, float &t) {
 	AffineLocal a;
-	affineLocal(rec[0], rec[1], rec[2], o, d, a);
+	affineLocal(rec[0], rec[1], rec[2], ray, a);
 	BoxSolve r;
 	float worldT;
 	if (!affineBoxSolve(a, r, worldT)) return false;
@@ -355,9 +458,9 @@ __device__ __forceinline__ bool testAffineBox(const float4 *rec, f3 o, f3 d, flo
 	return !(worldT >= limit);
 }
 
-// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.
-__device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 d,
-                                               float beforeT) {
+// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds, IEEE division.
+__device__ __forceinline__ float entryDistanceIeee(float lx, float ly, float lz, float hx, float hy, float hz, f3 o,
+                                                   f3 d, float beforeT) {
 	float tMin = kEps;
 	float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
 	float tMax = cb;
@@ -377,6 +480,41 @@ __device__ __forceinline__ float entryDistance(float lx, float ly, float lz, flo
 	MRAD_ENTRY_AXIS(o.z, d.z, lz, hz)
 #undef MRAD_ENTRY_AXIS
 	return tMin <= cb ? tMin : INFINITY;
+}
+// The same distance by the ray's reciprocals.  A zero quotient may come out
+// with the other sign here (0 / b by the reciprocal form); no branch of the
+// distance tells +0 from -0 (tMin >= 1e-4 > 0 and tMax > 0 before every axis),
+// so the zero case needs no select.
+__device__ __forceinline__ float entryDistanceFast(float lx, float ly, float lz, float hx, float hy, float hz,
+                                                   const Ray &r, float beforeT) {
+	float tMin = kEps;
+	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
+	float tMax = cb;
+#define MRAD_ENTRY_AXIS(OC, DC, INV, LO, HI)                                       \
+	if (DC == 0.0f) {                                                              \
+		if (OC < LO || OC > HI) return INFINITY;                                   \
+	} else {                                                                       \
+		const float n0 = LO - OC, n1 = HI - OC;                                    \
+		const float q0 = n0 * INV, q1 = n1 * INV;                                  \
+		float t0 = fmaf(fmaf(-q0, DC, n0), INV, q0);                               \
+		float t1 = fmaf(fmaf(-q1, DC, n1), INV, q1);                               \
+		if (t0 > t1) { const float tmp = t0; t0 = t1; t1 = tmp; }                  \
+		if (t1 > 0.0f) t1 *= 1.00000072f;                                          \
+		tMin = fmaxf(tMin, t0);                                                    \
+		tMax = fminf(tMax, t1);                                                    \
+		if (tMax < tMin) return INFINITY;                                          \
+	}
+	MRAD_ENTRY_AXIS(r.o.x, r.d.x, r.inv.x, lx, hx)
+	MRAD_ENTRY_AXIS(r.o.y, r.d.y, r.inv.y, ly, hy)
+	MRAD_ENTRY_AXIS(r.o.z, r.d.z, r.inv.z, lz, hz)
+#undef MRAD_ENTRY_AXIS
+	return tMin <= cb ? tMin : INFINITY;
+}
+// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.
+__device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, const Ray &r,
+                                               float beforeT) {
+	if (r.fast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, beforeT);
+	return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, beforeT);
 }
 
 // ------------------------------------------------------------ closest hit
@@ -408,7 +546,7 @@ __device__ __forceinline__ int typeSlot(uint32_t type) {
 
 // tracePrimitiveRef with the reference's (t, order) acceptance
 // (GpuKernelSources.java:772-806).
-__device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_t type, f3 o, f3 d, Best &best) {
+__device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_t type, const Ray &ray, Best &best) {
 	const int index = (int) (ref & kRefIndexMask);
 	const float limit = nextafterf(best.t, INFINITY);
 	float t = 0.0f;
@@ -416,16 +554,16 @@ __device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_
 	int kind;
 	if (type == kRefBox) {
 		kind = kKindBox;
-		found = testBox(S.boxes[2 * index], S.boxes[2 * index + 1], o, d, limit, t);
+		found = testBox(S.boxes[2 * index], S.boxes[2 * index + 1], ray, limit, t);
 	} else if (type == kRefAffineSphere) {
 		kind = kKindAffineSphere;
-		found = testAffineSphere(S.affS + 3 * index, o, d, limit, t);
+		found = testAffineSphere(S.affS + 3 * index, ray, limit, t);
 	} else if (type == kRefAffineBox) {
 		kind = kKindAffineBox;
-		found = testAffineBox(S.affB + 3 * index, o, d, limit, t);
+		found = testAffineBox(S.affB + 3 * index, ray, limit, t);
 	} else {
 		kind = kKindSphere;
-		found = testSphere(S.spheres[index], o, d, limit, t);
+		found = testSphere(S.spheres[index], ray, limit, t);
 	}
 	const uint32_t rank = type | (uint32_t) index;
 	if (!found || t > best.t || (t == best.t && rank >= best.rank)) return;
@@ -467,39 +605,71 @@ struct TravStack {
 	}
 };
 
-// traceBvh (GpuKernelSources.java:844-961): same stack discipline, same
-// counters.  kCap is the storage capacity; the runtime limit R.stackLimit
-// (<= kCap) gives BVH_STACK_SIZE semantics.
-template <bool kMetrics, int kCap, int kShort = kShortStack>
-__device__ inline void traceBvh(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
-                                Counters<kMetrics> &c, LdsStack lds) {
-	if (R.refCount <= 0 || R.rootCount <= 0 || R.rootCount > 4) return;
+// traceBvh (GpuKernelSources.java:844-961) in three parts: bvhBegin inserts
+// the roots, traceBvh's loop pops and expands stack entries, bvhEnd runs the
+// overflow fallback.  Same stack discipline, same counters.  kCap is the
+// storage capacity; the runtime limit R.stackLimit (<= kCap) gives
+// BVH_STACK_SIZE semantics.
+template <int kCap, int kShort = kShortStack>
+struct BvhState {
 	TravStack<kCap, kShort> st;
-	st.lds = lds;
+	int size;
+	bool overflow;
+};
+
+template <bool kMetrics, int kCap, int kShort>
+__device__ __forceinline__ void bvhBegin(const SceneRange &R, const Ray &ray, const Best &best, Counters<kMetrics> &c,
+                                         BvhState<kCap, kShort> &b) {
+	b.size = 0;
+	b.overflow = false;
+	if (R.refCount <= 0 || R.rootCount <= 0 || R.rootCount > 4) return;
 	const int cap = R.stackLimit;
-	int size = 0;
-	bool overflow = false;
 	for (int r = 0; r < R.rootCount; r++) {
 		const int ri = R.roots[r];
-		if (ri < 0) { overflow = true; break; }
+		if (ri < 0) { b.overflow = true; break; }
 		c.add(1, 1);
 		const float *rb = R.rootBounds[r];
-		float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, d, best.t);
+		float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ray, best.t);
 		if (!isfinite(e)) continue;
-		if (size >= cap) { overflow = true; break; }
-		int pos = size;
-		while (pos > 0 && e >= st.entry(pos - 1)) {
-			st.set(pos, st.node(pos - 1), st.entry(pos - 1));
+		if (b.size >= cap) { b.overflow = true; break; }
+		int pos = b.size;
+		while (pos > 0 && e >= b.st.entry(pos - 1)) {
+			b.st.set(pos, b.st.node(pos - 1), b.st.entry(pos - 1));
 			pos--;
 		}
-		st.set(pos, ri, e);
-		size++;
-		c.maxStack(size);
+		b.st.set(pos, ri, e);
+		b.size++;
+		c.maxStack(b.size);
 	}
+}
+
+// The reference's stack-overflow fallback: every ref of the scene, linearly.
+template <bool kMetrics, int kCap, int kShort>
+__device__ __forceinline__ void bvhEnd(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
+                                       Counters<kMetrics> &c, const BvhState<kCap, kShort> &b) {
+	if (!b.overflow) return;
+	c.add(8, 1);
+	for (int i = R.refBase; i < R.refBase + R.refCount; i++) {
+		const uint32_t ref = S.refs[i];
+		c.add(2, 1);
+		c.add(3 + typeSlot(ref & kRefTypeMask), 1);
+		testRef(S, ref, ref & kRefTypeMask, ray, best);
+	}
+}
+
+template <bool kMetrics, int kCap, int kShort = kShortStack>
+__device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
+                                Counters<kMetrics> &c, LdsStack lds) {
+	BvhState<kCap, kShort> b;
+	b.st.lds = lds;
+	bvhBegin<kMetrics>(R, ray, best, c, b);
+	const int cap = R.stackLimit;
+	int size = b.size;
+	bool overflow = b.overflow;
 	while (!overflow && size > 0) {
 		--size;
-		const int node = st.node(size);
-		const float entry = st.entry(size);
+		const int node = b.st.node(size);
+		const float entry = b.st.entry(size);
 		const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 		if (entry > ch) continue;
 		const float4 *rec = S.nodes + 4 * (size_t) node;
@@ -518,46 +688,40 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, f3 o, f3
 				const uint32_t ref = S.refs[ri];
 				const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
 				c.add(3 + typeSlot(type), 1);
-				testRef(S, ref, type, o, d, best);
+				testRef(S, ref, type, ray, best);
 			}
 			continue;
 		}
 		c.add(10, 1);
-		const float4 a = rec[0], b = rec[1], cc = rec[2];
-		const float le = entryDistance(a.x, a.y, a.z, a.w, b.x, b.y, o, d, best.t);
-		const float re = entryDistance(b.z, b.w, cc.x, cc.y, cc.z, cc.w, o, d, best.t);
+		const float4 a = rec[0], bb = rec[1], cc = rec[2];
+		const float le = entryDistance(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
+		const float re = entryDistance(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
 		c.add(1, 2);
 		int nearN = nd.x, farN = nd.y;
 		float nearE = le, farE = re;
 		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
 		if (isfinite(farE)) {
 			if (size >= cap) { overflow = true; break; }
-			st.set(size, farN, farE);
+			b.st.set(size, farN, farE);
 			size++;
 			c.maxStack(size);
 		}
 		if (isfinite(nearE)) {
 			if (size >= cap) { overflow = true; break; }
-			st.set(size, nearN, nearE);
+			b.st.set(size, nearN, nearE);
 			size++;
 			c.maxStack(size);
 		}
 	}
-	if (overflow) {
-		c.add(8, 1);
-		for (int i = R.refBase; i < R.refBase + R.refCount; i++) {
-			const uint32_t ref = S.refs[i];
-			c.add(2, 1);
-			c.add(3 + typeSlot(ref & kRefTypeMask), 1);
-			testRef(S, ref, ref & kRefTypeMask, o, d, best);
-		}
-	}
+	b.size = size;
+	b.overflow = overflow;
+	bvhEnd<kMetrics>(S, R, ray, best, c, b);
 }
 
 // Linear scan of every primitive class in global order, strict against the
 // running best (GpuKernelSources.java:993-1013).
 template <bool kMetrics>
-__device__ inline void traceLinear(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
+__device__ inline void traceLinear(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
                                    Counters<kMetrics> &c) {
 	c.add(2, (uint32_t) (R.sphCount + R.boxCount + R.asCount + R.abCount));
 	c.add(3, (uint32_t) R.sphCount);
@@ -566,14 +730,14 @@ __device__ inline void traceLinear(const DevScene &S, const SceneRange &R, f3 o,
 	c.add(7, (uint32_t) R.abCount);
 	float t;
 	for (int i = R.sphBase; i < R.sphBase + R.sphCount; i++)
-		if (testSphere(S.spheres[i], o, d, best.t, t)) best = {t, kRefSphere | (uint32_t) i, kKindSphere, i};
+		if (testSphere(S.spheres[i], ray, best.t, t)) best = {t, kRefSphere | (uint32_t) i, kKindSphere, i};
 	for (int i = R.boxBase; i < R.boxBase + R.boxCount; i++)
-		if (testBox(S.boxes[2 * i], S.boxes[2 * i + 1], o, d, best.t, t)) best = {t, kRefBox | (uint32_t) i, kKindBox, i};
+		if (testBox(S.boxes[2 * i], S.boxes[2 * i + 1], ray, best.t, t)) best = {t, kRefBox | (uint32_t) i, kKindBox, i};
 	for (int i = R.asBase; i < R.asBase + R.asCount; i++)
-		if (testAffineSphere(S.affS + 3 * i, o, d, best.t, t))
+		if (testAffineSphere(S.affS + 3 * i, ray, best.t, t))
 			best = {t, kRefAffineSphere | (uint32_t) i, kKindAffineSphere, i};
 	for (int i = R.abBase; i < R.abBase + R.abCount; i++)
-		if (testAffineBox(S.affB + 3 * i, o, d, best.t, t)) best = {t, kRefAffineBox | (uint32_t) i, kKindAffineBox, i};
+		if (testAffineBox(S.affB + 3 * i, ray, best.t, t)) best = {t, kRefAffineBox | (uint32_t) i, kKindAffineBox, i};
 }
 
 // tracePlane (GpuKernelSources.java:708-734) distance part.
@@ -625,16 +789,17 @@ struct HitDetail { f3 pos, normal; int matIndex; };
 __device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, float t, int kind, int i) {
 	HitDetail h;
 	h.pos = o + d * t;
+	const Ray ray = makeRay(o, d, false);
 	if (kind == kKindSphere) {
 		SphereSolve r;
-		sphereSolve(S.spheres[i], o, d, r);
+		sphereSolve(S.spheres[i], ray, r);
 		h.normal = normalizeSafe(mk(fmaf(d.x, r.radialOffset, r.closest.x), fmaf(d.y, r.radialOffset, r.closest.y),
 		                            fmaf(d.z, r.radialOffset, r.closest.z)));
 		h.matIndex = S.sphereMat[i];
 	} else if (kind == kKindBox) {
 		float4 b1 = S.boxes[2 * i + 1];
 		BoxSolve r;
-		boxSolve(S.boxes[2 * i], b1, o, d, r);
+		boxSolve(S.boxes[2 * i], b1, ray, r);
 		h.normal = axisNormal(r.axis, r.sign);
 		h.matIndex = __float_as_int(b1.z);
 	} else if (kind == kKindAffineSphere || kind == kKindAffineBox) {
@@ -642,7 +807,7 @@ __device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, float t, i
 		const float4 *rec = (isBox ? S.affB : S.affS) + 3 * i;
 		const float4 *nrm = (isBox ? S.affBN : S.affSN) + 3 * i;
 		AffineLocal a;
-		affineLocal(rec[0], rec[1], rec[2], o, d, a);
+		affineLocal(rec[0], rec[1], rec[2], ray, a);
 		f3 local;
 		if (isBox) {
 			BoxSolve r;
@@ -676,8 +841,9 @@ __device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange
 	best.rank = 0xFFFFFFFFu;
 	best.kind = -1;
 	best.index = -1;
-	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort>(S, R, o, d, best, c, lds);
-	else traceLinear<kMetrics>(S, R, o, d, best, c);
+	const Ray ray = makeRay(o, d, R.fastSlabs != 0);
+	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort>(S, R, ray, best, c, lds);
+	else traceLinear<kMetrics>(S, R, ray, best, c);
 	c.add(2, (uint32_t) R.planeCount);
 	c.add(5, (uint32_t) R.planeCount);
 	tracePlanes(S, R, o, d, best);

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -77,6 +78,22 @@ struct HostArena {
 	std::vector<int> sphereMat;
 	std::vector<uint32_t> refs;
 	std::vector<float> planes;
+
+	// Whether every node and box bound is 0 or of magnitude in [2^-35, 2^58],
+	// the condition under which the device takes slab quotients from the
+	// ray's reciprocals (see the ray constants in mrad_device.hpp).
+	static bool slabBoundsQualify(const mrad_scene_arrays *a) {
+		auto ok = [](float v) {
+			const float m = std::fabs(v);
+			return v == 0.0f || (m >= 0x1p-35f && m <= 0x1p58f);
+		};
+		for (size_t i = 0; i < 6 * (size_t) a->node_count; i++)
+			if (!ok(a->node_bounds[i])) return false;
+		for (int i = 0; i < a->box_count; i++)
+			for (int k = 0; k < 6; k++)
+				if (!ok(a->boxes[7 * (size_t) i + k])) return false;
+		return true;
+	}
 
 	// Appends one scene; fills its SceneRange.  useBvh and stackLimit come
 	// from the caller's configuration.
@@ -187,6 +204,7 @@ struct HostArena {
 		R.planeBase = (int) planeBase; R.planeCount = a->plane_count;
 		R.useBvh = (useBvh && nc > 0) ? 1 : 0;
 		R.stackLimit = stackLimit;
+		R.fastSlabs = slabBoundsQualify(a) ? 1 : 0;
 		for (int k = 0; k < 3; k++) R.bg[k] = a->background[k];
 		return MRAD_OK;
 	}

@@ -63,12 +63,18 @@ struct RenderParams {
 	unsigned long long *stats;
 };
 
+// Occupancy the megakernel is compiled for (waves per SIMD).
+#ifndef MRAD_RENDER_WAVES_PER_EU
+#define MRAD_RENDER_WAVES_PER_EU 6
+#endif
+
 // renderKernel (GpuKernelSources.java:1167-1315).  One thread per pixel runs
 // samples [s0, s1) of the frame and keeps the running sum in registers; the
 // float sum order and the last-sample normalization are the reference's.
 // A wave covers an 8x8 pixel block for primary-ray coherence.
 template <bool kMetrics, int kCap>
-__global__ __launch_bounds__(kBlock) void renderKernel(RenderParams P) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_RENDER_WAVES_PER_EU)))
+void renderKernel(RenderParams P) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int tile = xcdContiguous(blockIdx.x, gridDim.x);
 	const int tileX = tile % P.tilesX, tileY = tile / P.tilesX;

@@ -326,7 +326,8 @@ constexpr int kRegroupPerLane = 8;                       // refs per lane per ro
 constexpr int kTaskCap = 64 * kRegroupPerLane;
 
 struct RegroupLds {
-	float ray[6][64];
+	float ray[11][64];      // o, d, 1/d, dot(d, d), 1/dot(d, d); then the fast flag
+	uint8_t fast[64];
 	unsigned long long key[64];
 	float limitT[64];
 	uint32_t taskRef[kTaskCap];
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 				valid = false;
 			}
 		}
-		f3 o = mk(0.0f, 0.0f, 0.0f), d = mk(0.0f, 0.0f, 1.0f);
+		Ray ray = makeRay(mk(0.0f, 0.0f, 0.0f), mk(0.0f, 0.0f, 1.0f), false);
 		const SceneRange *Rp = P.scenes ? &P.scenes[valid ? (meta.z & 0xFFFFu) : 0].R : nullptr;
 		Best best;
 		best.t = 1e30f;
@@ -388,12 +389,11 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 		bool overflow = false, trav = false;
 		if (valid) {
 			const float4 a = Q.od0[i], b = Q.od1[i];
-			o = mk(a.x, a.y, a.z);
-			d = mk(a.w, b.x, b.y);
+			ray = makeRay(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), Rp->fastSlabs != 0);
 			c.add(0, 1);
 			const SceneRange &R = *Rp;
 			if (R.useBvh == 0) {
-				traceLinear<kMetrics>(P.S, R, o, d, best, c);
+				traceLinear<kMetrics>(P.S, R, ray, best, c);
 			} else if (R.refCount > 0 && R.rootCount > 0 && R.rootCount <= 4) {
 				trav = true;
 				for (int r = 0; r < R.rootCount; r++) {     // root insertion, as traceBvh
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 					if (ri < 0) { overflow = true; break; }
 					c.add(1, 1);
 					const float *rb = R.rootBounds[r];
-					float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, d, best.t);
+					float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ray, best.t);
 					if (!isfinite(e)) continue;
 					if (size >= R.stackLimit) { overflow = true; break; }
 					int pos = size;
@@ -415,8 +415,11 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 				}
 			}
 		}
-		G.ray[0][lane] = o.x; G.ray[1][lane] = o.y; G.ray[2][lane] = o.z;
-		G.ray[3][lane] = d.x; G.ray[4][lane] = d.y; G.ray[5][lane] = d.z;
+		G.ray[0][lane] = ray.o.x; G.ray[1][lane] = ray.o.y; G.ray[2][lane] = ray.o.z;
+		G.ray[3][lane] = ray.d.x; G.ray[4][lane] = ray.d.y; G.ray[5][lane] = ray.d.z;
+		G.ray[6][lane] = ray.inv.x; G.ray[7][lane] = ray.inv.y; G.ray[8][lane] = ray.inv.z;
+		G.ray[9][lane] = ray.dd; G.ray[10][lane] = ray.invDD;
+		G.fast[lane] = ray.fast ? 1 : 0;
 		for (;;) {
 			// traverse interior nodes until this lane reaches a leaf or finishes
 			bool atLeaf = false;
@@ -442,8 +445,8 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 				}
 				c.add(10, 1);
 				const float4 a = rec[0], b = rec[1], cc = rec[2];
-				const float le = entryDistance(a.x, a.y, a.z, a.w, b.x, b.y, o, d, best.t);
-				const float re = entryDistance(b.z, b.w, cc.x, cc.y, cc.z, cc.w, o, d, best.t);
+				const float le = entryDistance(a.x, a.y, a.z, a.w, b.x, b.y, ray, best.t);
+				const float re = entryDistance(b.z, b.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
 				c.add(1, 2);
 				int nearN = nd.x, farN = nd.y;
 				float nearE = le, farE = re;
@@ -507,16 +510,21 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 						if (j < n) {
 							const uint32_t ref = G.taskRef[off + j];
 							const int owner = G.taskOwner[off + j];
-							const f3 ro = mk(G.ray[0][owner], G.ray[1][owner], G.ray[2][owner]);
-							const f3 rd = mk(G.ray[3][owner], G.ray[4][owner], G.ray[5][owner]);
+							Ray rr;
+							rr.o = mk(G.ray[0][owner], G.ray[1][owner], G.ray[2][owner]);
+							rr.d = mk(G.ray[3][owner], G.ray[4][owner], G.ray[5][owner]);
+							rr.inv = mk(G.ray[6][owner], G.ray[7][owner], G.ray[8][owner]);
+							rr.dd = G.ray[9][owner];
+							rr.invDD = G.ray[10][owner];
+							rr.fast = G.fast[owner] != 0;
 							const float limit = nextafterf(G.limitT[owner], INFINITY);
 							const int index = (int) (ref & kRefIndexMask);
 							float tc = 0.0f;
 							bool found;
-							if (t == 0) found = testSphere(P.S.spheres[index], ro, rd, limit, tc);
-							else if (t == 1) found = testBox(P.S.boxes[2 * index], P.S.boxes[2 * index + 1], ro, rd, limit, tc);
-							else if (t == 2) found = testAffineSphere(P.S.affS + 3 * index, ro, rd, limit, tc);
-							else found = testAffineBox(P.S.affB + 3 * index, ro, rd, limit, tc);
+							if (t == 0) found = testSphere(P.S.spheres[index], rr, limit, tc);
+							else if (t == 1) found = testBox(P.S.boxes[2 * index], P.S.boxes[2 * index + 1], rr, limit, tc);
+							else if (t == 2) found = testAffineSphere(P.S.affS + 3 * index, rr, limit, tc);
+							else found = testAffineBox(P.S.affB + 3 * index, rr, limit, tc);
 							if (found) atomicMin(&G.key[owner], bestKey(tc, ref));
 						}
 					}
@@ -545,13 +553,13 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 					const uint32_t ref = P.S.refs[r];
 					c.add(2, 1);
 					c.add(3 + typeSlot(ref & kRefTypeMask), 1);
-					testRef(P.S, ref, ref & kRefTypeMask, o, d, best);
+					testRef(P.S, ref, ref & kRefTypeMask, ray, best);
 				}
 			}
 			c.add(2, (uint32_t) R.planeCount);
 			c.add(5, (uint32_t) R.planeCount);
-			tracePlanes(P.S, R, o, d, best);
-			writeHit(P, i, o, d, best, best.kind >= 0);
+			tracePlanes(P.S, R, ray.o, ray.d, best);
+			writeHit(P, i, ray.o, ray.d, best, best.kind >= 0);
 		}
 	}
 	if (P.stats) flushCounters<kMetrics>(c, P.stats);

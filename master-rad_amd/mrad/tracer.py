@@ -56,6 +56,24 @@ class TraceReplayResult:
     per_ray_counters: np.ndarray | None
 
 
+@dataclass(frozen=True)
+class ConditioningStats:
+    """ConditioningStats (GpuRayTracer.java:101-117): the diagnostics of a
+    conditioning sample, validated like the reference record."""
+    physical_launches: int
+    upload_nanos: int
+    aggregate_physical_kernel_nanos: int
+    maximum_physical_kernel_nanos: int
+    total_nanos: int
+
+    def __post_init__(self):
+        if (self.physical_launches < 1 or self.upload_nanos < 0 or self.aggregate_physical_kernel_nanos <= 0
+                or self.maximum_physical_kernel_nanos <= 0
+                or self.maximum_physical_kernel_nanos > self.aggregate_physical_kernel_nanos
+                or self.total_nanos <= 0):
+            raise N.InvalidArgument(N.MRAD_ERR_INVALID_ARGUMENT, "Invalid conditioning diagnostics")
+
+
 def validate_full_path_depth(max_depth: int) -> None:
     """GpuRayTracer.validateFullPathDepth (GpuRayTracer.java:470-476)."""
     N.check(N.lib.mrad_validate_full_path_depth(int(max_depth)))
@@ -143,6 +161,29 @@ class GpuRayTracer:
         cam = camera.packed() if isinstance(camera, GpuCamera) else np.ascontiguousarray(camera, np.float32)
         N.check(N.lib.mrad_render_sample(self._handle(), N.ptr(rgb, N.fp), N.ptr(cam, N.fp), int(max_depth),
                                          int(frame_index), C.c_uint64(frame_seed & ((1 << 64) - 1))))
+
+    def condition_steady_state_sample(self, scene: GpuScene, camera: GpuCamera, max_depth: int, frame_index: int,
+                                      frame_seed: int) -> ConditioningStats:
+        """conditionSteadyStateSample (GpuRayTracer.java:434-453): one sample
+        into a private buffer (uploading the scene if needed), its diagnostics."""
+        if getattr(self, "_conditioning", None) is None:
+            self._conditioning = np.zeros(self.width * self.height * 3, np.float32)
+        self.render_sample(self._conditioning, scene, camera, max_depth, frame_index, frame_seed)
+        st = self.last_frame_stats()
+        return ConditioningStats(st["physical_launches"], st["upload_nanos"], st["kernel_nanos"],
+                                 st["maximum_physical_kernel_nanos"], st["total_nanos"])
+
+    def render_steady_state_sample(self, rgb: np.ndarray | None, scene: GpuScene, camera: GpuCamera, max_depth: int,
+                                   frame_index: int, frame_seed: int) -> None:
+        """renderSteadyStateSample (GpuRayTracer.java:455-467): a sample that
+        must find the scene resident."""
+        self.render_sample(rgb, scene, camera, max_depth, frame_index, frame_seed)
+        if self.last_frame_stats()["upload_nanos"] != 0:
+            raise N.IllegalState(N.MRAD_ERR_ILLEGAL_STATE, "A steady-state sample uploaded scene data")
+
+    def last_physical_kernel_launch_count(self) -> int:
+        """lastPhysicalKernelLaunchCount (GpuRayTracer.java:294-296)."""
+        return int(self.last_frame_stats()["physical_launches"])
 
     def copy_frame(self, rgb: np.ndarray) -> None:
         N.check(N.lib.mrad_copy_frame(self._handle(), N.ptr(rgb, N.fp)))

@@ -81,3 +81,13 @@ def test_argument_checks_precede_device_checks():
 def test_destroy_null_is_noop():
     N.lib.mrad_destroy(None)
     N.lib.mrad_scene_destroy(None)
+
+
+def test_conditioning_stats_validation():
+    """ConditioningStats rejects the diagnostics GpuRayTracer.java:108-116 rejects."""
+    ok = mrad.ConditioningStats(2, 0, 100, 60, 500)
+    assert ok.physical_launches == 2
+    for bad in [(0, 0, 100, 60, 500), (1, -1, 100, 60, 500), (1, 0, 0, 0, 500), (1, 0, 100, 0, 500),
+                (1, 0, 100, 101, 500), (1, 0, 100, 60, 0)]:
+        with pytest.raises(mrad.InvalidArgument):
+            mrad.ConditioningStats(*bad)

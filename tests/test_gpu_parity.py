@@ -122,3 +122,60 @@ def test_full_size_field_sampled_pixels(require_gpu):
                              w.spp, pixels=pix, rr_start=w.rr_start)
     got = img.reshape(-1, 3)[pix]
     np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_steady_state_samples(require_gpu):
+    """conditionSteadyStateSample uploads and reports valid diagnostics; a steady-state sample of the resident
+    scene renders the same frame; one that would upload another scene raises IllegalState."""
+    scene, cam = configs.room_scene(), configs.room_camera()
+    other = small_field(500)
+    with mrad.GpuRayTracer(32, 24, 4, True, samples_per_launch=2) as t:
+        cs = t.condition_steady_state_sample(scene, cam, 4, 0, 11)
+        assert isinstance(cs, mrad.ConditioningStats)
+        assert cs.physical_launches == 2 == t.last_physical_kernel_launch_count()
+        assert cs.upload_nanos > 0
+        a = np.zeros(32 * 24 * 3, np.float32)
+        t.render_steady_state_sample(a, scene, cam, 4, 0, 11)
+        b = np.zeros_like(a)
+        t.render_sample(b, scene, cam, 4, 0, 11)
+        np.testing.assert_array_equal(a, b)
+        with pytest.raises(mrad.IllegalState):
+            t.render_steady_state_sample(a, other, cam, 4, 0, 11)
+
+
+def _quotient_edge_rays(scene, rng, n=6000):
+    """Rays whose slab quotients leave the fast reciprocal path's range or hit its edge cases: direction
+    components from 1e-40 (subnormal) to 1e30 and exact zeros, origins on node bound planes (zero numerators)
+    and origins with tiny coordinates (subnormal-scale numerators)."""
+    o = rng.uniform(-1.2, 1.2, (n, 3))
+    mag = 10.0 ** rng.uniform(-40, 30, (n, 3))
+    d = np.where(rng.random((n, 3)) < 0.5, -mag, mag)
+    d[rng.random((n, 3)) < 0.1] = 0.0
+    d[np.all(d == 0, axis=1), 0] = 1.0
+    bounds = scene.bvh_node_bounds().reshape(-1)
+    on_plane = rng.random((n, 3)) < 0.2
+    o[on_plane] = rng.choice(bounds, on_plane.sum())
+    tiny = rng.random((n, 3)) < 0.1
+    o[tiny] = rng.uniform(-1, 1, tiny.sum()) * 10.0 ** rng.uniform(-38, -15, tiny.sum())
+    # aim half of the rays at sphere centres so that many traverse deeply and hit
+    aim = rng.random(n) < 0.5
+    centres = scene.sphere_data()[:, 0:3].astype(np.float64)
+    tgt = centres[rng.integers(0, len(centres), aim.sum())]
+    d[aim] = (tgt - o[aim]) * 10.0 ** rng.uniform(-25, 25, (aim.sum(), 1))
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["field3k", "clustered4k"])
+def test_replay_quotient_edge_cases(require_gpu, name):
+    """The reciprocal-and-correction quotients of the slab and sphere tests equal IEEE division on every
+    ray, including those that take the division fallback (bit-exact hits and per-ray counters)."""
+    make_scene, _ = SCENES[name]
+    scene = make_scene()
+    rays = _quotient_edge_rays(scene, np.random.default_rng(77))
+    assert np.all(np.isfinite(rays))
+    print(f"quotient edge rays {name} n={len(rays)}", flush=True)
+    ref = O.trace_replay(O.Scene(scene.wire()), rays)
+    with mrad.GpuRayTracer(8, 8, 1, True, collect_metrics=True) as t:
+        res = t.trace_replay(scene, rays, collect_metrics=True, per_ray_counters=True)
+    assert_replay_equal(res, ref)
+    assert int(ref[0].sum()) > 1000     # enough hits to exercise the primitive tests
