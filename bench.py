@@ -155,6 +155,15 @@ def bench_batch(args) -> dict:
     log(f"built 64 scenes ({prims} primitives, {total_paths} paths) in {time.perf_counter() - t0:.1f} s")
     cfg = dict(queue_capacity=args.queue_capacity, waves_per_cu=args.waves_per_cu)
     b = make_batch(ws, **cfg)
+    tuned = None
+    if args.autotune:
+        from mrad.autotune import tune
+        t1 = time.perf_counter()
+        best, table = tune(b)
+        cfg = dict(queue_capacity=best.queue_capacity, waves_per_cu=best.waves_per_cu,
+                   lds_stack_entries=best.lds_stack_entries)
+        tuned = {"best": cfg, "trials": table, "seconds": time.perf_counter() - t1}
+        log(f"autotuned in {tuned['seconds']:.0f} s: {cfg}")
     for _ in range(args.warmup):
         b.render()
     wall, kern = [], []
@@ -164,7 +173,10 @@ def bench_batch(args) -> dict:
         wall.append(time.perf_counter() - t1)
         kern.append(st["kernel_nanos"] / 1e9)
         log(f"wavefront step {i}: {wall[-1] * 1e3:.0f} ms, {st['steps']} bounce steps")
+    t1 = time.perf_counter()
     wf_images = [b.image(k) for k in range(len(ws))]
+    copy_s = time.perf_counter() - t1
+    image_bytes = sum(img.nbytes for img in wf_images)
     b.close()
     value = total_paths * args.steps / sum(wall) / 1e6
 
@@ -218,12 +230,15 @@ def bench_batch(args) -> dict:
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "batch-64 wavefront", "scenes": 64, "primitives": prims, "paths_per_step": total_paths,
                    "resolutions": "256^2..1920x1080", "spp": "4/16/64", "max_depth": 6, "russian_roulette_after": 3,
-                   "queue_capacity": args.queue_capacity, "waves_per_cu": args.waves_per_cu, "hipgraph": True},
+                   **cfg, "hipgraph": True},
+        "autotune": tuned,
         "megakernel_mpaths_per_s": mk_value,
         "wavefront_over_megakernel": value / mk_value,
         "kernel_only": {"wavefront_ms": float(np.mean(kern)) * 1e3, "megakernel_ms": mk_kern * 1e3,
                         "ratio": mk_kern / float(np.mean(kern))},
         "images_bit_identical": identical,
+        "host_copy": {"image_bytes": image_bytes, "seconds": copy_s,
+                      "pcie_inclusive_mpaths_per_s": total_paths / (sum(wall) / args.steps + copy_s) / 1e6},
         "relmse_equal": identical,
         "mrays_per_s": st_m["rays"] / (sum(wall) / args.steps) / 1e6,
         "segments_per_path": st_m["rays"] / total_paths,
@@ -344,6 +359,8 @@ def main():
     ap.add_argument("--queue-capacity", type=int, default=1 << 23)
     ap.add_argument("--waves-per-cu", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune queue capacity, waves per CU and the LDS split on the batch before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: every frame and batch renders on one GPU and this build has no "
