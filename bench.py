@@ -20,7 +20,9 @@ every scene resident in HBM and every image left in HBM.  Reported beside it:
 --workload het (100k primitives, 1080p, 64 spp, 8 bounces) and --workload
 clustered (1M clustered primitives, half affine, 1080p, 16 spp, 6 bounces) time
 one full-size frame in the megakernel, the wavefront and the regrouped
-wavefront forms.
+wavefront forms.  --workload hero times the reference's own Layered Hero
+frames (96 to 1M primitives, 1080p, 1 spp, 12 bounces) under the uniform and
+weighted SAH.
 """
 from __future__ import annotations
 
@@ -350,12 +352,78 @@ def bench_single(args, w: configs.Workload) -> dict:
     }, [w]
 
 
+def bench_hero(args) -> dict:
+    """The reference's own Layered Hero timing frames (LayeredHeroFamily, 96 to
+    1M primitives; 1920x1080, 1 spp, depth 12, stack 32) in the megakernel,
+    uniform and weighted (lambda 1) SAH at leaf 8.  Reported per scale: kernel
+    ms per frame (median), segments per second, and the weighted-vs-uniform
+    kernel-time reduction beside the reference's published one (RTX 3080,
+    core8-detail.csv).  The line's value is the 1M-primitive weighted frame."""
+    from mrad import workloads as W
+    pub = {(r["rowId"], r["method"]): r
+           for r in json.loads((ROOT / "tests" / "golden" / "hero_counters.json").read_text())}
+    frames = max(5, 10 * args.steps)
+    cam = W.hero_gpu_camera()
+    rows, line = [], None
+    for scale in W.HERO_SCALES:
+        rid = W.scale_id(scale)
+        key = W.workload_key(rid)
+        res = {}
+        for method, lam in (("uniform", 0.0), ("weighted", 1.0)):
+            cfg = mrad.BvhBuildConfig.uniform(W.LEAF_SIZE) if lam == 0.0 else mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, lam)
+            sc = W.hero_scene(scale, cfg)
+            base = W.timing_seed(key, 0)
+            with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
+                for i in range(max(1, args.warmup)):
+                    t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
+                kern = []
+                t0 = time.perf_counter()
+                for i in range(frames):
+                    t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
+                    kern.append(t.last_frame_stats()["kernel_nanos"])
+                wall = (time.perf_counter() - t0) / frames
+            with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, collect_metrics=True,
+                                   bvh_stack_size=W.STACK_CAPACITY) as tm:
+                tm.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, 0, W.metrics_seed(key))
+                st = tm.last_frame_stats()
+            k_ms = float(np.median(kern)) / 1e6
+            res[method] = {"kernel_ms": k_ms, "wall_ms": wall * 1e3, "segments": st["rays"],
+                           "gsegments_per_s": st["rays"] / (k_ms * 1e6),
+                           "roofline_gbs": (algorithmic_bytes(st) + 12.0 * W.WIDTH * W.HEIGHT) / (k_ms * 1e6),
+                           "counters": st}
+            log(f"{rid} {method}: {k_ms:.3f} ms kernel, {wall * 1e3:.3f} ms wall, {st['rays']} segments")
+        red = 100.0 * (1.0 - res["weighted"]["kernel_ms"] / res["uniform"]["kernel_ms"])
+        rows.append({"row": rid, "primitives": scale + 9,
+                     "uniform_kernel_ms": res["uniform"]["kernel_ms"], "weighted_kernel_ms": res["weighted"]["kernel_ms"],
+                     "weighted_gsegments_per_s": res["weighted"]["gsegments_per_s"],
+                     "weighted_time_reduction_pct": red,
+                     "published_rtx3080_time_reduction_pct": pub[(rid, "weighted")]["timeReductionPercent"]})
+        if scale == W.HERO_SCALES[-1]:
+            line = dict(res["weighted"], scene=sc)
+    paths = W.WIDTH * W.HEIGHT
+    return {
+        "metric": METRIC, "value": paths / (line["wall_ms"] / 1e3) / 1e6, "unit": "Mpaths/s", "n_gpus": 1,
+        "steps": frames, "warmup": args.warmup, "ms_per_step": line["wall_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (the reference's deterministic Layered Hero generator)",
+        "config": {"workload": "hero-1m weighted megakernel (LayeredHeroFamily 1M + 9 environment primitives)",
+                   "width": W.WIDTH, "height": W.HEIGHT, "spp": 1, "max_depth": W.MAXIMUM_PATH_DEPTH,
+                   "bvh": "weighted SAH lambda 1 leaf 8", "stack": W.STACK_CAPACITY},
+        "mrays_per_s": line["segments"] / (line["wall_ms"] / 1e3) / 1e6,
+        "scales": rows,
+        "roofline": {"bound": "hbm", "achieved": line["roofline_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": line["roofline_gbs"] / HBM_PEAK_GBS, "traffic": load_traffic("hero-1m"),
+                     "kernel": "renderKernel", "kernel_ms_per_step": line["kernel_ms"]},
+        "counters_per_step": {k: line["counters"][k] for k in mrad.STAT_NAMES},
+    }, [configs.Workload("hero-1m", line["scene"], cam, W.WIDTH, W.HEIGHT, 1, W.MAXIMUM_PATH_DEPTH)]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["batch", "het", "clustered"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "het", "clustered", "hero"], default="batch")
     ap.add_argument("--queue-capacity", type=int, default=1 << 23)
     ap.add_argument("--waves-per-cu", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -370,6 +438,9 @@ def main():
         res, ws = bench_batch(args)
         # the largest 1080p scene of the batch
         sample = max((w for w in ws if w.width == 1920), key=lambda w: w.scene.primitive_count())
+    elif args.workload == "hero":
+        res, ws = bench_hero(args)
+        sample = ws[0]
     else:
         res, ws = bench_single(args, configs.het(100_000) if args.workload == "het" else configs.clustered())
         sample = ws[0]

@@ -1,0 +1,364 @@
+"""The reference's fixed benchmark workloads: Layered Hero scenes, their camera
+and the metrics-frame seed, so that whole frames of the reference's own
+evaluation can be rendered here and their per-frame counters compared with
+the published ones.
+
+  LayeredHeroFamily.create/camera   graphics3d/scene/v1/LayeredHeroFamily.java:55-136
+  SceneSupport.sphere/box/camera    graphics3d/scene/v1/SceneSupport.java:42-89
+  Affine composition and rotations  graphics3d/Affine.java:35-41, :114-136
+  BenchmarkRays.gpuCamera           benchmark/BenchmarkRays.java:28-66
+  BenchmarkProtocol.metricsSeed     benchmark/BenchmarkProtocol.java:295-297, :385-400
+  BenchmarkWorkloads.fixed          benchmark/BenchmarkWorkloads.java:48-70
+
+All arithmetic is IEEE double in the reference's operation order (Python
+floats are doubles and evaluate left to right like Java), so the packed FP32
+scene is the reference's bit for bit.  Angles of the reference's Affine
+rotations are in turns; their sine and cosine are taken as sin(2 pi t) and
+cos(2 pi t) of the C library.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .scene import BvhBuildConfig, GpuScene, material_array
+from .tracer import GpuCamera
+
+# ---- BenchmarkProtocol (benchmark/BenchmarkProtocol.java:39-46, :62-63)
+WIDTH, HEIGHT = 1920, 1080
+SAMPLES_PER_PIXEL = 1
+MAXIMUM_PATH_DEPTH = 12
+STACK_CAPACITY = 32
+LEAF_SIZE = 8
+WEIGHTED_LAMBDA = 1.0
+HERO_SCALES = (96, 1_000, 10_000, 100_000, 1_000_000)
+HERO_LEAF_SIZES = (1, 2, 4, 8, 16, 32, 64)
+HERO_LAMBDAS = (0.0, 0.25, 0.5, 0.75, 1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0)
+PUBLICATION_ROW_IDS = ("hero-96", "hero-1k", "hero-10k", "hero-100k", "hero-1m", "overlap-10k")
+
+_M64 = (1 << 64) - 1
+_MIX = 0x9E3779B97F4A7C15
+
+
+def _rotl(v: int, r: int) -> int:
+    return ((v << r) | (v >> (64 - r))) & _M64
+
+
+def seed_domain(label: str) -> int:
+    """BenchmarkProtocol.seedDomain (:402-408): Java String-hash style, 64-bit wrap."""
+    v = 1_125_899_906_842_597
+    for ch in label:
+        v = (31 * v + ord(ch)) & _M64
+    return v
+
+
+def _seed(namespace: int, first: int, second: int) -> int:
+    """BenchmarkProtocol.seed (:385-393); returns the unsigned 64-bit value."""
+    v = namespace ^ _rotl(((first + 1) * _MIX) & _M64, 17) ^ _rotl(((second + 1) * _MIX) & _M64, 41)
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & _M64
+    v ^= v >> 31
+    return namespace if v == 0 else v
+
+
+METRICS_SEED_DOMAIN = seed_domain("evaluation metrics")
+TIMING_SEED_DOMAIN = seed_domain("evaluation timing")
+
+
+def workload_key(row_id: str) -> int:
+    """BenchmarkProtocol.workloadKey (:279-289) of a fixed (Hero or overlap) row: its ordinal."""
+    return PUBLICATION_ROW_IDS.index(row_id)
+
+
+def metrics_seed(key: int) -> int:
+    """BenchmarkProtocol.metricsSeed (:295-297): the frame seed of the counter frame."""
+    return _seed(METRICS_SEED_DOMAIN, key, 0)
+
+
+def timing_seed(key: int, comparison: int) -> int:
+    return _seed(TIMING_SEED_DOMAIN, key, comparison)
+
+
+# ---- Affine (graphics3d/Affine.java), rows (m00..m03, m10..m13, m20..m23)
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+
+
+def then(a, t):
+    """Affine.then (:35-41): first a, then t (t * a), in the reference's summation order."""
+    m00, m01, m02, m03, m10, m11, m12, m13, m20, m21, m22, m23 = a
+    t00, t01, t02, t03, t10, t11, t12, t13, t20, t21, t22, t23 = t
+    return (t00 * m00 + t01 * m10 + t02 * m20, t00 * m01 + t01 * m11 + t02 * m21,
+            t00 * m02 + t01 * m12 + t02 * m22, t00 * m03 + t01 * m13 + t02 * m23 + t03,
+            t10 * m00 + t11 * m10 + t12 * m20, t10 * m01 + t11 * m11 + t12 * m21,
+            t10 * m02 + t11 * m12 + t12 * m22, t10 * m03 + t11 * m13 + t12 * m23 + t13,
+            t20 * m00 + t21 * m10 + t22 * m20, t20 * m01 + t21 * m11 + t22 * m21,
+            t20 * m02 + t21 * m12 + t22 * m22, t20 * m03 + t21 * m13 + t22 * m23 + t23)
+
+
+def sin_t(turns: float) -> float:
+    return math.sin(turns * 2.0 * math.pi)
+
+
+def cos_t(turns: float) -> float:
+    return math.cos(turns * 2.0 * math.pi)
+
+
+def translation(x, y, z):
+    return (1.0, 0.0, 0.0, x, 0.0, 1.0, 0.0, y, 0.0, 0.0, 1.0, z)
+
+
+def scaling(x, y, z):
+    return (x, 0.0, 0.0, 0.0, 0.0, y, 0.0, 0.0, 0.0, 0.0, z, 0.0)
+
+
+def rotation_about_x(t):
+    c, s = cos_t(t), sin_t(t)
+    return (1.0, 0.0, 0.0, 0.0, 0.0, c, -s, 0.0, 0.0, s, c, 0.0)
+
+
+def rotation_about_y(t):
+    c, s = cos_t(t), sin_t(t)
+    return (c, 0.0, s, 0.0, 0.0, 1.0, 0.0, 0.0, -s, 0.0, c, 0.0)
+
+
+def rotation_about_z(t):
+    c, s = cos_t(t), sin_t(t)
+    return (c, -s, 0.0, 0.0, s, c, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+
+
+def _at(a, v):
+    m00, m01, m02, m03, m10, m11, m12, m13, m20, m21, m22, m23 = a
+    x, y, z = v
+    return (m00 * x + m01 * y + m02 * z + m03, m10 * x + m11 * y + m12 * z + m13, m20 * x + m21 * y + m22 * z + m23)
+
+
+def _linear(a, v):
+    m00, m01, m02, _, m10, m11, m12, _, m20, m21, m22, _ = a
+    x, y, z = v
+    return (m00 * x + m01 * y + m02 * z, m10 * x + m11 * y + m12 * z, m20 * x + m21 * y + m22 * z)
+
+
+# ---- cameras (cameras/Perspective.java, cameras/TransformedCamera.java, SceneSupport.camera :64-77)
+class Camera:
+    """TransformedCamera(Perspective(k), transform): the sensor point (sx, sy)
+    leaves the origin along (sx, sy, 1/k), then the transform is applied."""
+
+    def __init__(self, k: float, transform=IDENTITY):
+        self.k = k
+        self.transform = transform
+
+    def ray(self, sx: float, sy: float):
+        d = (sx, sy, 1 / self.k)
+        return _at(self.transform, (0.0, 0.0, 0.0)), _linear(self.transform, d)
+
+
+def support_camera(position, pitch: float, yaw: float, roll: float = 0.0) -> Camera:
+    t = then(then(then(then(IDENTITY, translation(*position)), rotation_about_x(pitch)), rotation_about_y(yaw)),
+             rotation_about_z(roll))
+    return Camera(1.0 / 3.0, t)
+
+
+def _sensor(x: float, y: float, width: int, height: int):
+    aspect = width / height
+    return ((-1.0 + 2.0 * x / width) * aspect, 1.0 - 2.0 * y / height)
+
+
+def gpu_camera(camera: Camera, width: int, height: int) -> GpuCamera:
+    """BenchmarkRays.gpuCamera (:28-46): the linear camera through the centre pixel
+    and its right and upper neighbours."""
+    cx, cy = (width - 1) / 2.0, (height - 1) / 2.0
+    c = _sensor(cx + 0.5, cy + 0.5, width, height)
+    r = _sensor(min(width - 1, cx + 1.0) + 0.5, cy + 0.5, width, height)
+    u = _sensor(cx + 0.5, min(height - 1, cy + 1.0) + 0.5, width, height)
+    cp, cd = camera.ray(*c)
+    _, rd = camera.ray(*r)
+    _, ud = camera.ray(*u)
+    right = tuple((a - b) / (r[0] - c[0]) for a, b in zip(rd, cd))
+    up = tuple((a - b) / (u[1] - c[1]) for a, b in zip(ud, cd))
+    fwd = tuple(d - rr * c[0] - uu * c[1] for d, rr, uu in zip(cd, right, up))
+    return GpuCamera(cp, fwd, right, up)
+
+
+# ---- Layered Hero (graphics3d/scene/v1/LayeredHeroFamily.java)
+PRIMITIVES_PER_MODULE = 8
+_DEPTH_SEPARATION, _BOX_HALF_LENGTH, _BOX_DEPTH_TILT_TURNS = 0.750, 2.000, 0.150
+_ASPECT = 16.0 / 9.0
+_SENSOR_HALF_HEIGHT = 0.92
+_SENSOR_HALF_WIDTH = _ASPECT * _SENSOR_HALF_HEIGHT
+_PERSPECTIVE_Z = 3.0
+_LAYER_OFFSET = 0.28
+_MODULE_SCALE = 0.35
+_DEPTH_STEP = 2.35
+_OUTER_X, _OUTER_Y, _MIDDLE_Y = 0.72, 0.72, 0.76
+_SPHERE_RADIUS = 0.150
+_BOX_HALF_WIDTH, _BOX_HALF_DEPTH = 0.065, 0.110
+_DIAGONAL_TURNS = 0.125
+_BOX_NORMAL_OFFSET = 0.140
+
+
+def _surface(emission, diffuse=(0.0, 0.0, 0.0), reflective=(0.0, 0.0, 0.0), refractive=(0.0, 0.0, 0.0), ior=1.5):
+    """SceneSupport.surface -> Material.surface (Material.java:86-88) in packed order."""
+    return material_array(diffuse, reflective, refractive, emission, ior)[0]
+
+
+# Palette(green, cyan, blue, red, orange) (:44-49)
+_GREEN = _surface((0.015, 0.56, 0.36))
+_CYAN = _surface((0.015, 0.34, 0.78))
+_BLUE = _surface((0.13, 0.48, 0.98))
+_RED = _surface((1.52, 0.045, 0.008))
+_ORANGE = _surface((1.28, 0.25, 0.010))
+
+
+class _Grid:
+    """LayeredHeroFamily.Grid (:226-273)."""
+
+    def __init__(self, modules: int):
+        nominal = max(1, math.ceil(math.sqrt(modules * _ASPECT)))
+        self.rows = max(1, (modules + nominal - 1) // nominal)
+        self.base_columns = modules // self.rows
+        self.extra_rows = modules % self.rows
+        self.cell_w = 2.0 * _SENSOR_HALF_WIDTH / nominal
+        self.cell_h = 2.0 * _SENSOR_HALF_HEIGHT / self.rows
+
+    def _centred_extra(self, row: int) -> bool:
+        rows, extra = self.rows, self.extra_rows
+        if extra <= 0:
+            return False
+        if rows & 1:
+            dist = abs(row - rows // 2)
+            return dist <= extra // 2 if extra & 1 else 0 < dist <= extra // 2
+        pairs = extra // 2
+        lower, upper = rows // 2 - pairs, rows // 2 + pairs
+        return lower <= row < upper if (extra & 1) == 0 else lower <= row <= upper
+
+    def columns_in_row(self, row: int) -> int:
+        return self.base_columns + (1 if self._centred_extra(row) else 0)
+
+    def sensor_x(self, column: int, in_row: int) -> float:
+        return (column - (in_row - 1) * 0.5) * self.cell_w
+
+    def sensor_y(self, row: int) -> float:
+        return _SENSOR_HALF_HEIGHT - (row + 0.5) * self.cell_h
+
+
+def _world(sx, sy, depth):
+    return (sx * depth / _PERSPECTIVE_Z, sy * depth / _PERSPECTIVE_Z, depth)
+
+
+def hero_primitives(primitive_count: int):
+    """LayeredHeroFamily.create (:55-86) as typed lists in the reference's
+    insertion order: spheres (centre, radius, material), axis-aligned boxes
+    (min, max, material), affine boxes (forward 3x4 transform, material) and
+    the background."""
+    if primitive_count <= 0 or primitive_count % PRIMITIVES_PER_MODULE:
+        raise ValueError(f"primitiveCount must be a positive multiple of 8: {primitive_count}")
+    modules = primitive_count // PRIMITIVES_PER_MODULE
+    grid = _Grid((modules + 2) // 3)
+    spheres, sphere_mats, aboxes, abox_mats = [], [], [], []
+    layer_cos = [math.cos((layer / 3.0 - 0.25) * 2.0 * math.pi) for layer in range(3)]
+    layer_sin = [math.sin((layer / 3.0 - 0.25) * 2.0 * math.pi) for layer in range(3)]
+    rot = {}
+    for sign in (1, -1):
+        base = sign * _DIAGONAL_TURNS
+        radians = base * 2.0 * math.pi
+        axis_x, axis_y = -math.sin(radians), math.cos(radians)
+        rot[sign] = (base, axis_y, -axis_x)        # baseRotation, normalX, normalY
+
+    def sphere(sx, sy, scale, depth, mat):
+        spheres.append((*_world(sx, sy, depth), _SPHERE_RADIUS * scale * (depth / _PERSPECTIVE_Z)))
+        sphere_mats.append(mat)
+
+    module = 0
+    for row in range(grid.rows):
+        if module >= modules:
+            break
+        in_row = grid.columns_in_row(row)
+        for column in range(in_row):
+            if module >= modules:
+                break
+            cell_x, cell_y = grid.sensor_x(column, in_row), grid.sensor_y(row)
+            cell_scale = min(grid.cell_w, grid.cell_h)
+            for layer in range(3):
+                if module >= modules:
+                    break
+                sx = cell_x + _LAYER_OFFSET * cell_scale * layer_cos[layer]
+                sy = cell_y + _LAYER_OFFSET * cell_scale * layer_sin[layer]
+                scale = _MODULE_SCALE * cell_scale
+                depth = 4.8 + layer * _DEPTH_STEP
+                sign = 1 if ((row + column + layer) & 1) == 0 else -1
+                # addModule (:138-167)
+                half_sep = 0.5 * _DEPTH_SEPARATION * (scale * depth / _PERSPECTIVE_Z)
+                cheap, box_depth = depth - half_sep, depth + half_sep
+                sphere(sx - _OUTER_X * scale, sy + _OUTER_Y * scale, scale, cheap, _GREEN)
+                sphere(sx, sy + _MIDDLE_Y * scale, scale, cheap, _BLUE)
+                sphere(sx + _OUTER_X * scale, sy + _OUTER_Y * scale, scale, cheap, _CYAN)
+                sphere(sx - _OUTER_X * scale, sy - _OUTER_Y * scale, scale, cheap, _CYAN)
+                sphere(sx, sy - _MIDDLE_Y * scale, scale, cheap, _GREEN)
+                sphere(sx + _OUTER_X * scale, sy - _OUTER_Y * scale, scale, cheap, _BLUE)
+                # addBoxPair (:169-204)
+                base, nx, ny = rot[sign]
+                for box in range(2):
+                    s = -1.0 if box == 0 else 1.0
+                    bx = sx + s * scale * _BOX_NORMAL_OFFSET * nx
+                    by = sy + s * scale * _BOX_NORMAL_OFFSET * ny
+                    screen = base + (0.0 if box == 0 else 0.5)
+                    unit = scale * box_depth / _PERSPECTIVE_Z
+                    t = then(IDENTITY, scaling(_BOX_HALF_WIDTH * unit, _BOX_HALF_LENGTH * unit, _BOX_HALF_DEPTH * unit))
+                    t = then(t, rotation_about_x(s * _BOX_DEPTH_TILT_TURNS))
+                    t = then(t, rotation_about_z(screen))
+                    t = then(t, translation(*_world(bx, by, box_depth)))
+                    aboxes.append(t)
+                    abox_mats.append(_RED if box == 0 else _ORANGE)
+                module += 1
+    if len(spheres) + len(aboxes) != primitive_count:
+        raise RuntimeError("Layered module count mismatch")
+    # addRayTracingEnvironment(solids, 12.8) (:88-118)
+    rear = 12.8
+    mirror = _surface((0.002, 0.003, 0.006), reflective=(0.82, 0.88, 0.98))
+    lens = _surface((0.004, 0.010, 0.018), reflective=(0.06, 0.08, 0.10), refractive=(0.88, 0.96, 1.00), ior=1.33)
+    frame = _surface((0.025, 0.22, 0.72), diffuse=(0.010, 0.035, 0.10), reflective=(0.22, 0.30, 0.46))
+    fz, fs = rear - 0.10, rear / 8.8
+    boxes, box_mats = [], []
+    for centre, radii, mat in (
+            ((0.0, 0.0, rear + 0.045), (2.66 * fs, 1.57 * fs, 0.045), mirror),
+            ((-2.72 * fs, 0.0, fz), (0.055 * fs, 1.68 * fs, 0.045), frame),
+            ((2.72 * fs, 0.0, fz), (0.055 * fs, 1.68 * fs, 0.045), frame),
+            ((0.0, 1.68 * fs, fz), (2.78 * fs, 0.055 * fs, 0.045), frame),
+            ((0.0, -1.68 * fs, fz), (2.78 * fs, 0.055 * fs, 0.045), frame)):
+        # SceneBox.axisAligned -> Box.fromCenterAndRadii: p = c - r, q = c + r
+        boxes.append((*(c - r for c, r in zip(centre, radii)), *(c + r for c, r in zip(centre, radii))))
+        box_mats.append(mat)
+    for x in (-2.15, 2.15):
+        for y in (-1.18, 1.18):
+            spheres.append((x, y, 4.20, 0.22))
+            sphere_mats.append(lens)
+    return {"spheres": (np.array(spheres, np.float64), np.array(sphere_mats)),
+            "boxes": (np.array(boxes, np.float64), np.array(box_mats)),
+            "affine_boxes": (np.array(aboxes, np.float64), np.array(abox_mats)),
+            "background": (0.002, 0.004, 0.014)}
+
+
+def hero_scene(primitive_count: int, config: BvhBuildConfig | None = None) -> GpuScene:
+    """GpuSceneBuilder.from(LayeredHeroFamily.create(n), config) (gpu/GpuSceneBuilder.java:27-77)."""
+    p = hero_primitives(primitive_count)
+    sc = GpuScene(config or BvhBuildConfig.uniform(LEAF_SIZE))
+    sc.set_background(p["background"])
+    sc.add_spheres(*p["spheres"])
+    sc.add_boxes(*p["boxes"])
+    sc.add_affine_transforms("box", *p["affine_boxes"])
+    return sc
+
+
+def hero_camera() -> Camera:
+    """LayeredHeroFamily.camera (:120-122)."""
+    return support_camera((0.0, 0.0, 0.0), 0.0, 0.0)
+
+
+def hero_gpu_camera(width: int = WIDTH, height: int = HEIGHT) -> GpuCamera:
+    return gpu_camera(hero_camera(), width, height)
+
+
+def scale_id(scale: int) -> str:
+    """BenchmarkProtocol.compact (:410-417) as used in the row ids."""
+    return "hero-" + {1_000: "1k", 10_000: "10k", 100_000: "100k", 1_000_000: "1m"}.get(scale, str(scale))

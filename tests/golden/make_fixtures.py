@@ -3,6 +3,13 @@ result files (data only; run in a container where the reference is mounted).
 
   gi_test_topology.json  <- benchmarks/results/main-bvh-study/leaf-size-time.csv
                             (GI_TEST rows: leaf size, topology SHA-256, nodes, depth)
+  hero_topology.json     <- benchmarks/results/main-bvh-study/hero-sensitivity.csv
+                            (Layered Hero: scale, leaf size, lambda, topology SHA-256)
+                            and core8-detail.csv (uniform / weighted rows at leaf 8)
+  hero_counters.json     <- benchmarks/results/main-bvh-study/core8-detail.csv
+                            (per-frame counters of the uniform and weighted rows of
+                            the Hero scales: 1920x1080, 1 spp, depth 12, stack 32;
+                            and the weighted-vs-uniform kernel-time reduction)
 """
 import csv
 import json
@@ -11,11 +18,17 @@ from pathlib import Path
 
 REF = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
 OUT = Path(__file__).resolve().parent
+STUDY = REF / "benchmarks/results/main-bvh-study"
+SCALES = {"hero-96": 96, "hero-1k": 1_000, "hero-10k": 10_000, "hero-100k": 100_000, "hero-1m": 1_000_000}
+COUNTERS = {"rays": "medianRays", "aabb_tests": "medianAabbTests", "primitive_tests": "medianPrimitiveTests",
+            "sphere_tests": "medianSphereTests", "box_tests": "medianBoxTests", "plane_tests": "medianPlaneTests",
+            "affine_sphere_tests": "medianAffineSphereTests", "affine_box_tests": "medianAffineBoxTests",
+            "internal_node_visits": "medianInternalNodeVisits", "leaf_node_visits": "medianLeafNodeVisits"}
 
 
 def gi_test_topology():
     rows = []
-    with open(REF / "benchmarks/results/main-bvh-study/leaf-size-time.csv") as f:
+    with open(STUDY / "leaf-size-time.csv") as f:
         for r in csv.DictReader(f):
             if r["sceneId"] == "GI_TEST":
                 rows.append({"leafSize": int(r["leafSize"]), "topologySha256": r["topologySha256"],
@@ -23,5 +36,35 @@ def gi_test_topology():
     (OUT / "gi_test_topology.json").write_text(json.dumps(rows, indent=1) + "\n")
 
 
+def _core8_hero_rows():
+    with open(STUDY / "core8-detail.csv") as f:
+        return [r for r in csv.DictReader(f) if r["rowId"] in SCALES and r["methodId"] in ("uniform", "weighted")]
+
+
+def hero_topology():
+    cells = []
+    with open(STUDY / "hero-sensitivity.csv") as f:
+        for r in csv.DictReader(f):
+            cells.append({"rowId": r["rowId"], "scale": int(r["scale"]), "leafSize": int(r["leafSize"]),
+                          "lambda": float(r["lambda"]), "topologySha256": r["topologySha256"]})
+    for r in _core8_hero_rows():
+        cells.append({"rowId": r["rowId"], "scale": SCALES[r["rowId"]], "leafSize": 8,
+                      "lambda": 0.0 if r["methodId"] == "uniform" else 1.0, "topologySha256": r["topologySha256"]})
+    (OUT / "hero_topology.json").write_text(json.dumps(cells, indent=0) + "\n")
+
+
+def hero_counters():
+    rows = []
+    for r in _core8_hero_rows():
+        rows.append({"rowId": r["rowId"], "rowOrdinal": int(r["rowOrdinal"]), "scale": SCALES[r["rowId"]],
+                     "method": r["methodId"], "layoutCount": int(r["layoutCount"]),
+                     "topologySha256": r["topologySha256"],
+                     "timeReductionPercent": float(r["timeReductionPercent"]),
+                     "counters": {k: int(r[c]) for k, c in COUNTERS.items()}})
+    (OUT / "hero_counters.json").write_text(json.dumps(rows, indent=1) + "\n")
+
+
 if __name__ == "__main__":
     gi_test_topology()
+    hero_topology()
+    hero_counters()

@@ -1,0 +1,51 @@
+"""GPU: the reference's own Layered Hero metrics frames (1920x1080, 1 spp,
+depth 12, stack 32, frame seed BenchmarkProtocol.metricsSeed), rendered by
+the megakernel and by the wavefront form, against the oracle bit for bit:
+every pixel's float bits and all 14 frame counters.  These scenes exercise
+the mirror, glass and emissive lobes and thousands of affine boxes."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mrad
+from mrad import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("hero-96", 96, 0.0), ("hero-10k", 10_000, 1.0), ("hero-100k", 100_000, 1.0)]
+
+
+def _cfg(lam):
+    return mrad.BvhBuildConfig.uniform(8) if lam == 0.0 else mrad.BvhBuildConfig.weighted(8, lam)
+
+
+@pytest.mark.parametrize("row,scale,lam", CASES, ids=[c[0] for c in CASES])
+def test_hero_metrics_frame_bit_exact(require_gpu, row, scale, lam):
+    sc = W.hero_scene(scale, _cfg(lam))
+    cam = W.hero_gpu_camera()
+    seed = W.metrics_seed(W.workload_key(row))
+    print(f"hero frame {row} lambda={lam} seed={seed:#x}", flush=True)
+    ref, ref_st = O.render_pixels(O.Scene(sc.wire(), stack_size=W.STACK_CAPACITY), cam.packed(), W.WIDTH, W.HEIGHT,
+                                  W.MAXIMUM_PATH_DEPTH, 0, seed, 1)
+    ref_stats = dict(zip(mrad.STAT_NAMES, map(int, ref_st)))
+    img = np.zeros(W.WIDTH * W.HEIGHT * 3, np.float32)
+    with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, W.SAMPLES_PER_PIXEL, True, collect_metrics=True,
+                           bvh_stack_size=W.STACK_CAPACITY) as t:
+        t.render_sample(img, sc, cam, W.MAXIMUM_PATH_DEPTH, 0, seed)
+        st = t.last_frame_stats()
+    np.testing.assert_array_equal(img.view(np.uint32), ref.reshape(-1).view(np.uint32))
+    for k in mrad.STAT_NAMES:
+        assert st[k] == ref_stats[k], k
+
+    # the same frame in the wavefront form, with and without counters
+    for metrics in (True, False):
+        cfg = mrad.WavefrontConfig(queue_capacity=1 << 20, collect_metrics=metrics, bvh_stack_size=W.STACK_CAPACITY)
+        with mrad.BatchRenderer(cfg) as b:
+            b.add_scene(sc, cam, W.WIDTH, W.HEIGHT, 1, W.MAXIMUM_PATH_DEPTH, 0, seed)
+            bst = b.render()
+            np.testing.assert_array_equal(b.image(0).view(np.uint32), ref.reshape(-1).view(np.uint32))
+        if metrics:
+            for k in mrad.STAT_NAMES:
+                assert bst[k] == ref_stats[k], k
