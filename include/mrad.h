@@ -253,6 +253,25 @@ int mrad_batch_configure(mrad_batch *batch, const mrad_wavefront_config *config)
 int mrad_batch_copy_image(mrad_batch *batch, int scene, float *rgb_out);
 int mrad_batch_get_stats(mrad_batch *batch, mrad_batch_stats *out);
 
+/* ------------------------------------------- primitive-cost calibration */
+/* The measurement kernel of the reference's calibration campaign
+ * (gpu/PrimitiveCostBenchmark.java:717-873) on this library's
+ * traversal-time operations, for the weighted SAH's per-primitive weights
+ * (gpu/PrimitiveCostModel.java:11-19).  The workload uses the reference's
+ * calibration layout (PrimitiveCostBenchmark.java:488-580): profiles x records
+ * rays (6 floats), then records spheres (5), boxes (7), affine records (22),
+ * planes (26), node AABBs (6), node pairs (12); records a power of two. */
+typedef struct mrad_cost mrad_cost;
+/* compute_units receives the device's CU count (for sizing the grid). */
+int mrad_cost_create(int device, const float *workload, int records, int profiles, int *compute_units,
+                     mrad_cost **out);
+/* One timed launch (PrimitiveCostBenchmark.runChunk, :158-199): operation 0
+ * setup, 1 sphere, 2 box, 3 affine sphere, 4 affine box, 5 node AABB,
+ * 6 interior traversal, 7 plane; elapsed_ms from HIP events around it. */
+int mrad_cost_time(mrad_cost *cost, int operation, int profile, int iteration_offset, int iterations, int copies,
+                   int blocks, int threads, float *elapsed_ms);
+void mrad_cost_destroy(mrad_cost *cost);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,7 +34,7 @@ DEVICE_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
-HIP_SOURCES = ["mrad_runtime.hip", "mrad_wavefront.hip", "mrad_adjoint.hip"]
+HIP_SOURCES = ["mrad_runtime.hip", "mrad_wavefront.hip", "mrad_adjoint.hip", "mrad_calibrate.hip"]
 CPP_SOURCES = ["scene_builder.cpp"]
 
 

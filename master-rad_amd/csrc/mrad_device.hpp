@@ -545,8 +545,8 @@ __device__ __forceinline__ int typeSlot(uint32_t type) {
 }
 
 // tracePrimitiveRef with the reference's (t, order) acceptance
-// (GpuKernelSources.java:772-806).
-__device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_t type, const Ray &ray, Best &best) {
+// (GpuKernelSources.java:772-806) on the primitive's test record `rec`.
+__device__ __forceinline__ void testRefAt(const float4 *rec, uint32_t ref, uint32_t type, const Ray &ray, Best &best) {
 	const int index = (int) (ref & kRefIndexMask);
 	const float limit = nextafterf(best.t, INFINITY);
 	float t = 0.0f;
@@ -554,20 +554,52 @@ __device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_
 	int kind;
 	if (type == kRefBox) {
 		kind = kKindBox;
-		found = testBox(S.boxes[2 * index], S.boxes[2 * index + 1], ray, limit, t);
+		found = testBox(rec[0], rec[1], ray, limit, t);
 	} else if (type == kRefAffineSphere) {
 		kind = kKindAffineSphere;
-		found = testAffineSphere(S.affS + 3 * index, ray, limit, t);
+		found = testAffineSphere(rec, ray, limit, t);
 	} else if (type == kRefAffineBox) {
 		kind = kKindAffineBox;
-		found = testAffineBox(S.affB + 3 * index, ray, limit, t);
+		found = testAffineBox(rec, ray, limit, t);
 	} else {
 		kind = kKindSphere;
-		found = testSphere(S.spheres[index], ray, limit, t);
+		found = testSphere(rec[0], ray, limit, t);
 	}
 	const uint32_t rank = type | (uint32_t) index;
 	if (!found || t > best.t || (t == best.t && rank >= best.rank)) return;
 	best.t = t; best.rank = rank; best.kind = kind; best.index = index;
+}
+__device__ __forceinline__ const float4 *primitiveRecord(const DevScene &S, uint32_t ref, uint32_t type) {
+	const size_t index = ref & kRefIndexMask;
+	return type == kRefBox ? S.boxes + 2 * index : type == kRefAffineSphere ? S.affS + 3 * index
+	     : type == kRefAffineBox ? S.affB + 3 * index : S.spheres + index;
+}
+__device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_t type, const Ray &ray, Best &best) {
+	testRefAt(primitiveRecord(S, ref, type), ref, type, ray, best);
+}
+
+// The refs of a leaf node record `rec` with data `nd` = {kind, ref if one,
+// first ref, count}.  A one-ref leaf carries its primitive's test record in
+// the node record (rec[0..2]), so only the node itself is fetched.
+template <bool kMetrics>
+__device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R, const float4 *rec, int4 nd,
+                                         const Ray &ray, Best &best, Counters<kMetrics> &c) {
+	c.add(11, 1);
+	c.add(nd.x == -1 ? 13 : 12, 1);
+	c.add(2, (uint32_t) nd.w);
+	const int kind = nd.x;
+	const bool mixed = kind > -2 || kind < -5;
+	const uint32_t leafType = kind == -2 ? kRefSphere : kind == -3 ? kRefBox
+	                        : kind == -4 ? kRefAffineSphere : kRefAffineBox;
+	const bool inlined = nd.w == 1;
+	for (int i = 0; i < nd.w; i++) {
+		const int ri = nd.z + i;
+		if (!inlined && (ri < R.refBase || ri >= R.refBase + R.refCount)) continue;
+		const uint32_t ref = inlined ? (uint32_t) nd.y : S.refs[ri];
+		const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
+		c.add(3 + typeSlot(type), 1);
+		testRefAt(inlined ? rec : primitiveRecord(S, ref, type), ref, type, ray, best);
+	}
 }
 
 // Traversal stack storage.  The top kShortStack entries of each lane live in
@@ -575,6 +607,17 @@ __device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_
 // accesses hit 64 distinct banks whatever each lane's depth; deeper entries
 // spill to a thread-private array.  Capacity and overflow semantics are the
 // reference's BVH_STACK_SIZE (GpuKernelSources.java:858-859, :871, :924, :933).
+// Leaf batching of the traversal loop: 0 tests a leaf's refs as soon as a
+// lane pops it; K > 0 holds popped leaves until K lanes of the wave (or all
+// that still traverse) hold one.  Both visit nodes and test refs in the same
+// per-lane order, so hits and counters are identical.
+#ifndef MRAD_LEAF_BATCH
+#define MRAD_LEAF_BATCH 0
+#endif
+// 1 fetches a popped node's whole 64-byte record before its kind is known.
+#ifndef MRAD_NODE_PREFETCH
+#define MRAD_NODE_PREFETCH 0
+#endif
 #ifndef MRAD_SHORT_STACK
 #define MRAD_SHORT_STACK 8
 #endif
@@ -666,6 +709,59 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 	const int cap = R.stackLimit;
 	int size = b.size;
 	bool overflow = b.overflow;
+#if MRAD_LEAF_BATCH > 0
+	// The same per-lane sequence (pop, then expand an interior node or test a
+	// leaf's refs before the next pop), but a lane that pops a leaf holds it
+	// until MRAD_LEAF_BATCH lanes of the wave hold one (or every lane still
+	// traversing does), and those leaves are tested together: the leaf branch
+	// then runs with many active lanes instead of one or two.
+	bool active = !overflow && size > 0;
+	bool pending = false;
+	int4 leaf = make_int4(0, 0, 0, 0);
+	int leafNode = 0;
+	while (active) {
+		if (!pending) {
+			--size;
+			const int node = b.st.node(size);
+			const float entry = b.st.entry(size);
+			const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
+			if (entry <= ch) {
+				const float4 *rec = S.nodes + 4 * (size_t) node;
+				const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
+				if (nd.x < 0) {
+					leaf = nd;
+					leafNode = node;
+					pending = true;
+				} else {
+					c.add(10, 1);
+					const float4 a = rec[0], bb = rec[1], cc = rec[2];
+					const float le = entryDistance(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
+					const float re = entryDistance(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
+					c.add(1, 2);
+					int nearN = nd.x, farN = nd.y;
+					float nearE = le, farE = re;
+					if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+					if (isfinite(farE)) {
+						if (size >= cap) overflow = true;
+						else { b.st.set(size, farN, farE); size++; c.maxStack(size); }
+					}
+					if (!overflow && isfinite(nearE)) {
+						if (size >= cap) overflow = true;
+						else { b.st.set(size, nearN, nearE); size++; c.maxStack(size); }
+					}
+				}
+			}
+		}
+		const unsigned long long waiting = __ballot(pending);
+		if (waiting == __ballot(1) || __popcll(waiting) >= MRAD_LEAF_BATCH) {
+			if (pending) {
+				testLeaf<kMetrics>(S, R, S.nodes + 4 * (size_t) leafNode, leaf, ray, best, c);
+				pending = false;
+			}
+		}
+		active = !overflow && (pending || size > 0);
+	}
+#else
 	while (!overflow && size > 0) {
 		--size;
 		const int node = b.st.node(size);
@@ -673,27 +769,18 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 		if (entry > ch) continue;
 		const float4 *rec = S.nodes + 4 * (size_t) node;
+#if MRAD_NODE_PREFETCH
+		const float4 a = rec[0], bb = rec[1], cc = rec[2];
+#endif
 		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
 		if (nd.x < 0) {
-			c.add(11, 1);
-			c.add(nd.x == -1 ? 13 : 12, 1);
-			c.add(2, (uint32_t) nd.w);
-			const int kind = nd.x;
-			const bool mixed = kind > -2 || kind < -5;
-			const uint32_t leafType = kind == -2 ? kRefSphere : kind == -3 ? kRefBox
-			                        : kind == -4 ? kRefAffineSphere : kRefAffineBox;
-			for (int i = 0; i < nd.w; i++) {
-				const int ri = nd.z + i;
-				if (ri < R.refBase || ri >= R.refBase + R.refCount) continue;
-				const uint32_t ref = S.refs[ri];
-				const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
-				c.add(3 + typeSlot(type), 1);
-				testRef(S, ref, type, ray, best);
-			}
+			testLeaf<kMetrics>(S, R, rec, nd, ray, best, c);
 			continue;
 		}
 		c.add(10, 1);
+#if !MRAD_NODE_PREFETCH
 		const float4 a = rec[0], bb = rec[1], cc = rec[2];
+#endif
 		const float le = entryDistance(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
 		const float re = entryDistance(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
 		c.add(1, 2);
@@ -713,6 +800,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			c.maxStack(size);
 		}
 	}
+#endif
 	b.size = size;
 	b.overflow = overflow;
 	bvhEnd<kMetrics>(S, R, ray, best, c, b);

@@ -96,8 +96,23 @@ struct HostArena {
 	}
 
 	// Appends one scene; fills its SceneRange.  useBvh and stackLimit come
-	// from the caller's configuration.
+	// from the caller's configuration.  A scene that fails validation leaves
+	// the arena as it was.
 	int append(const mrad_scene_arrays *a, bool useBvh, int stackLimit, SceneRange &R) {
+		const size_t n0 = nodes.size(), s0 = spheres.size(), b0 = boxes.size(), as0 = affS.size(),
+		             asn0 = affSN.size(), ab0 = affB.size(), abn0 = affBN.size(), m0 = materials.size(),
+		             sm0 = sphereMat.size(), r0 = refs.size(), p0 = planes.size();
+		const int rc = appendScene(a, useBvh, stackLimit, R);
+		if (rc != MRAD_OK) {
+			nodes.resize(n0); spheres.resize(s0); boxes.resize(b0); affS.resize(as0); affSN.resize(asn0);
+			affB.resize(ab0); affBN.resize(abn0); materials.resize(m0); sphereMat.resize(sm0); refs.resize(r0);
+			planes.resize(p0);
+		}
+		return rc;
+	}
+
+  private:
+	int appendScene(const mrad_scene_arrays *a, bool useBvh, int stackLimit, SceneRange &R) {
 		const int nc = a->node_count;
 		if (a->sphere_count < 0 || a->box_count < 0 || a->plane_count < 0 || a->affine_sphere_count < 0 ||
 		    a->affine_box_count < 0 || nc < 0 || a->primitive_ref_count < 0 || a->root_count < 0 ||
@@ -119,31 +134,15 @@ struct HostArena {
 			if (s > kRefIndexMask) return set_error(MRAD_ERR_INVALID_ARGUMENT, "arena exceeds 2^30 records");
 		const int mb = (int) matBase;
 		auto mat = [mb](int m) { return m < 0 ? m : m + mb; };
-		// nodes: child pairs, indices rebased
-		nodes.resize(nodes.size() + (size_t) nc * 4);
-		for (int n = 0; n < nc; n++) {
-			const int32_t *nd = a->node_data + 4 * (size_t) n;
-			float4 *r = nodes.data() + 4 * (nodeBase + n);
-			if (nd[0] >= 0) {
-				if (nd[0] >= nc || nd[1] < 0 || nd[1] >= nc)
-					return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH child index out of range at node " + std::to_string(n));
-				const float *L = a->node_bounds + 6 * (size_t) nd[0];
-				const float *Rb = a->node_bounds + 6 * (size_t) nd[1];
-				r[0] = f4(L[0], L[1], L[2], L[3]);
-				r[1] = f4(L[4], L[5], Rb[0], Rb[1]);
-				r[2] = f4(Rb[2], Rb[3], Rb[4], Rb[5]);
-				r[3] = f4(ibits(nd[0] + (int) nodeBase), ibits(nd[1] + (int) nodeBase), ibits(nd[2]), ibits(nd[3]));
-			} else {
-				r[0] = r[1] = r[2] = f4(0, 0, 0, 0);
-				r[3] = f4(ibits(nd[0]), ibits(nd[1]), ibits(nd[2] + (int) refBase), ibits(nd[3]));
-			}
-		}
 		for (int i = 0; i < a->primitive_ref_count; i++) {
 			const uint32_t ref = (uint32_t) a->primitive_refs[i];
 			const uint32_t type = ref & kRefTypeMask;
 			const size_t base = type == kRefSphere ? sphBase : type == kRefBox ? boxBase : type == kRefAffineSphere ? asBase : abBase;
+			const int count = type == kRefSphere ? a->sphere_count : type == kRefBox ? a->box_count
+			                : type == kRefAffineSphere ? a->affine_sphere_count : a->affine_box_count;
+			if ((ref & kRefIndexMask) >= (uint32_t) count)
+				return set_error(MRAD_ERR_INVALID_ARGUMENT, "primitive ref " + std::to_string(i) + " out of range");
 			const size_t idx = (ref & kRefIndexMask) + base;
-			if (idx > kRefIndexMask) return set_error(MRAD_ERR_INVALID_ARGUMENT, "primitive ref index out of range");
 			refs.push_back(type | (uint32_t) idx);
 		}
 		for (int i = 0; i < a->sphere_count; i++) {
@@ -186,6 +185,40 @@ struct HostArena {
 			materials.push_back(f4(m[3], m[4], m[5], 0.0f));
 			materials.push_back(f4(m[6], m[7], m[8], 0.0f));
 			materials.push_back(f4(m[9], m[10], m[11], m[12]));
+		}
+		// nodes: child pairs, indices rebased.  A leaf of one ref carries that
+		// primitive's test record in the 48 bytes an interior node uses for its
+		// children's bounds, and the rebased ref in .y, so testing it needs no
+		// fetch beyond the node record itself.
+		nodes.resize(nodes.size() + (size_t) nc * 4);
+		for (int n = 0; n < nc; n++) {
+			const int32_t *nd = a->node_data + 4 * (size_t) n;
+			float4 *r = nodes.data() + 4 * (nodeBase + n);
+			if (nd[0] >= 0) {
+				if (nd[0] >= nc || nd[1] < 0 || nd[1] >= nc)
+					return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH child index out of range at node " + std::to_string(n));
+				const float *L = a->node_bounds + 6 * (size_t) nd[0];
+				const float *Rb = a->node_bounds + 6 * (size_t) nd[1];
+				r[0] = f4(L[0], L[1], L[2], L[3]);
+				r[1] = f4(L[4], L[5], Rb[0], Rb[1]);
+				r[2] = f4(Rb[2], Rb[3], Rb[4], Rb[5]);
+				r[3] = f4(ibits(nd[0] + (int) nodeBase), ibits(nd[1] + (int) nodeBase), ibits(nd[2]), ibits(nd[3]));
+			} else {
+				if (nd[3] < 0 || nd[2] < 0 || (int64_t) nd[2] + nd[3] > a->primitive_ref_count)
+					return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH leaf refs out of range at node " + std::to_string(n));
+				r[0] = r[1] = r[2] = f4(0, 0, 0, 0);
+				int y = nd[1];
+				if (nd[3] == 1) {
+					const uint32_t ref = refs[refBase + nd[2]];
+					const uint32_t idx = ref & kRefIndexMask, type = ref & kRefTypeMask;
+					const float4 *src = type == kRefSphere ? &spheres[idx] : type == kRefBox ? &boxes[2 * (size_t) idx]
+					                  : type == kRefAffineSphere ? &affS[3 * (size_t) idx] : &affB[3 * (size_t) idx];
+					const int words = type == kRefSphere ? 1 : type == kRefBox ? 2 : 3;
+					for (int k = 0; k < words; k++) r[k] = src[k];
+					y = (int) ref;
+				}
+				r[3] = f4(ibits(nd[0]), ibits(y), ibits(nd[2] + (int) refBase), ibits(nd[3]));
+			}
 		}
 		R = SceneRange{};
 		R.rootCount = a->root_count;

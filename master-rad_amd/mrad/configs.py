@@ -17,8 +17,10 @@ splitmix64(0x4D49333535580000 + k); batch scene i uses splitmix64(s_4 + i).
                (0.18, 0.22, 0.30).
   clustered    64 Gaussian clusters (sigma 0.05), 50 % affine, 50 % spheres,
                a ground box, 0.5 % emitters, BVH leaf 4.
-Every BVH is the weighted SAH (lambda 1) with the reference's calibrated
-weights; leaf 8 unless stated.
+The room's BVH is the weighted SAH (lambda 1) with the reference's calibrated
+weights; the seeded fields (het, clustered, batch, diff) use the weighted SAH
+(lambda 1) with the gfx950 weights of this library's kernels
+(scene.GFX950_WEIGHTS); leaf 8 unless stated.
 """
 from __future__ import annotations
 
@@ -96,7 +98,8 @@ def _transforms(rot: np.ndarray, scale: np.ndarray, centre: np.ndarray) -> np.nd
 
 
 def field_scene(seed: int, n: int, *, clustered: bool = False, leaf_size: int = 8, emissive_fraction: float = 0.01,
-                ground: bool = False, affine_fraction: float = 0.10, box_fraction: float = 0.30) -> GpuScene:
+                ground: bool = False, affine_fraction: float = 0.10, box_fraction: float = 0.30,
+                bvh: BvhBuildConfig | None = None) -> GpuScene:
     rng = np.random.default_rng(seed)
     n_aff = int(round(n * affine_fraction))
     n_box = int(round(n * box_fraction))
@@ -119,7 +122,7 @@ def field_scene(seed: int, n: int, *, clustered: bool = False, leaf_size: int = 
         emit[idx] = rng.uniform(5, 20, n_emit)[:, None]
     mats = _mat(albedo, emit)
     o = 0
-    sc = GpuScene(BvhBuildConfig.weighted(leaf_size))
+    sc = GpuScene(bvh or BvhBuildConfig.gfx950(leaf_size))
     sc.set_background((0.18, 0.22, 0.30))
     sl = slice(o, o + n_sph); o += n_sph
     sc.add_spheres(np.concatenate([centres[sl], radius[sl, None]], 1), mats[sl])
@@ -209,20 +212,22 @@ def batch_specs(count: int = 64, max_prims: int = 1_000_000) -> list[dict]:
     return specs
 
 
-def batch_scene(spec: dict) -> Workload:
-    sc = field_scene(spec["seed"], spec["prims"])
+def batch_scene(spec: dict, bvh: BvhBuildConfig | None = None) -> Workload:
+    sc = field_scene(spec["seed"], spec["prims"], bvh=bvh)
     sc.bvh_stats()          # packs and builds the BVH (the native build releases the GIL)
     return Workload(f"batch-{spec['index']}", sc, field_camera(), spec["width"], spec["height"], spec["spp"],
                     spec["max_depth"], spec["rr_start"], frame_seed=spec["seed"] & 0xFFFFFFFF)
 
 
-def batch(count: int = 64, threads: int = 16, max_prims: int = 1_000_000) -> list[Workload]:
-    """The 64-scene batch, scenes generated and their BVHs built on a thread pool."""
+def batch(count: int = 64, threads: int = 16, max_prims: int = 1_000_000,
+          bvh: BvhBuildConfig | None = None) -> list[Workload]:
+    """The 64-scene batch, scenes generated and their BVHs built on a thread pool
+    (`bvh` replaces the default weighted SAH, e.g. with calibrated weights)."""
     from concurrent.futures import ThreadPoolExecutor
     specs = batch_specs(count, max_prims)
     order = sorted(range(count), key=lambda i: -specs[i]["prims"])   # largest first for balance
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
-        built = dict(zip(order, ex.map(lambda i: batch_scene(specs[i]), order)))
+        built = dict(zip(order, ex.map(lambda i: batch_scene(specs[i], bvh), order)))
     return [built[i] for i in range(count)]
 
 

@@ -24,6 +24,15 @@ PLANE = 2.860319327
 NODE_AABB = 0.507189194
 INTERIOR_TRAVERSAL = 1.275576713
 
+# gfx950 (MI355X) weights of this library's kernels, from `python -m
+# mrad.calibrate --in-situ`: primitive classes by the reference's isolated
+# protocol (5 contexts), node AABB and interior traversal fitted in situ on
+# the trace kernel of the 64-scene batch (an AABB test costs 1/27 of a
+# primitive test there).
+GFX950_WEIGHTS = {"sphere": 1.000000000, "box": 1.296939838, "affineSphere": 1.633064197,
+                  "affineBox": 1.661005931, "plane": 1.311160810, "nodeAabb": 0.041857181,
+                  "interiorTraversal": 0.083714363}
+
 
 @dataclass(frozen=True)
 class BvhBuildConfig:
@@ -44,6 +53,14 @@ class BvhBuildConfig:
     @staticmethod
     def uniform(leaf_size: int = 8) -> "BvhBuildConfig":
         return BvhBuildConfig("UNIFORM_SAH", leaf_size)
+
+    @staticmethod
+    def gfx950(leaf_size: int = 8, lam: float = 1.0) -> "BvhBuildConfig":
+        """Weighted SAH with the gfx950 weights (GFX950_WEIGHTS)."""
+        w = GFX950_WEIGHTS
+        return BvhBuildConfig("WEIGHTED_SAH", leaf_size, traversal_weight=w["interiorTraversal"],
+                              sphere_weight=w["sphere"], box_weight=w["box"],
+                              affine_sphere_weight=w["affineSphere"], affine_box_weight=w["affineBox"], lam=lam)
 
     def native(self) -> N.BvhConfig:
         modes = {"UNIFORM_SAH": N.MRAD_BVH_UNIFORM_SAH, "WEIGHTED_SAH": N.MRAD_BVH_WEIGHTED_SAH}

@@ -37,7 +37,10 @@ def gi_camera() -> mrad.GpuCamera:
 
 
 def small_field(n: int = 3000, seed: int = 11, clustered: bool = False) -> mrad.GpuScene:
+    """A seeded field; the clustered one keeps the reference's weights (its
+    deeper tree overflows an 8-entry traversal stack), the other uses the
+    fields' default gfx950 weights."""
     if clustered:
         return configs.field_scene(seed, n, clustered=True, leaf_size=4, emissive_fraction=0.005, ground=True,
-                                   affine_fraction=0.5, box_fraction=0.0)
+                                   affine_fraction=0.5, box_fraction=0.0, bvh=mrad.BvhBuildConfig.weighted(4))
     return configs.field_scene(seed, n)
