@@ -510,9 +510,42 @@ __device__ __forceinline__ float entryDistanceFast(float lx, float ly, float lz,
 #undef MRAD_ENTRY_AXIS
 	return tMin <= cb ? tMin : INFINITY;
 }
-// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.
+// The reciprocal form for a ray with no zero direction component: the
+// reference's zero-direction branch of each axis never runs.
+__device__ __forceinline__ float entryDistanceFastNz(float lx, float ly, float lz, float hx, float hy, float hz,
+                                                     const Ray &r, float beforeT) {
+	float tMin = kEps;
+	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
+	float tMax = cb;
+#define MRAD_ENTRY_AXIS(OC, DC, INV, LO, HI)                                       \
+	{                                                                              \
+		const float n0 = LO - OC, n1 = HI - OC;                                    \
+		const float q0 = n0 * INV, q1 = n1 * INV;                                  \
+		float t0 = fmaf(fmaf(-q0, DC, n0), INV, q0);                               \
+		float t1 = fmaf(fmaf(-q1, DC, n1), INV, q1);                               \
+		if (t0 > t1) { const float tmp = t0; t0 = t1; t1 = tmp; }                  \
+		if (t1 > 0.0f) t1 *= 1.00000072f;                                          \
+		tMin = fmaxf(tMin, t0);                                                    \
+		tMax = fminf(tMax, t1);                                                    \
+		if (tMax < tMin) return INFINITY;                                          \
+	}
+	MRAD_ENTRY_AXIS(r.o.x, r.d.x, r.inv.x, lx, hx)
+	MRAD_ENTRY_AXIS(r.o.y, r.d.y, r.inv.y, ly, hy)
+	MRAD_ENTRY_AXIS(r.o.z, r.d.z, r.inv.z, lz, hz)
+#undef MRAD_ENTRY_AXIS
+	return tMin <= cb ? tMin : INFINITY;
+}
+// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.  kSlab picks
+// the quotient form: kSlabAny by the ray's flag, kSlabFast or kSlabIeee when
+// the caller already knows it (a traversal loop specialised per form keeps
+// one form on its hot path).
+constexpr int kSlabAny = 0, kSlabFast = 1, kSlabIeee = 2, kSlabFastNz = 3;
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, const Ray &r,
                                                float beforeT) {
+	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastNz(lx, ly, lz, hx, hy, hz, r, beforeT);
+	if constexpr (kSlab == kSlabFast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, beforeT);
+	if constexpr (kSlab == kSlabIeee) return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, beforeT);
 	if (r.fast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, beforeT);
 	return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, beforeT);
 }
@@ -614,6 +647,11 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 #ifndef MRAD_LEAF_BATCH
 #define MRAD_LEAF_BATCH 0
 #endif
+// 1 specialises the traversal loop per slab-quotient form (see traceClosest);
+// 2 adds a loop for reciprocal-form rays with no zero direction component.
+#ifndef MRAD_SPLIT_SLAB
+#define MRAD_SPLIT_SLAB 1
+#endif
 // 1 fetches a popped node's whole 64-byte record before its kind is known.
 #ifndef MRAD_NODE_PREFETCH
 #define MRAD_NODE_PREFETCH 0
@@ -623,9 +661,23 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 #endif
 constexpr int kShortStack = MRAD_SHORT_STACK;
 
+// The pointers are typed as LDS (address space 3).  With generic pointers
+// the compiler merges the LDS and spill reads of TravStack into one flat load
+// through a select, which is slower than a ds_read and waits for every
+// outstanding global load (megakernel 1.7 % slower on the same GPU).
+#ifndef MRAD_LDS_STACK_TYPED
+#define MRAD_LDS_STACK_TYPED 1
+#endif
+#if MRAD_LDS_STACK_TYPED
+typedef __attribute__((address_space(3))) int LdsInt;
+typedef __attribute__((address_space(3))) float LdsFloat;
+#else
+typedef int LdsInt;
+typedef float LdsFloat;
+#endif
 struct LdsStack {
-	int *node;     // lane base, stride 64
-	float *entry;
+	LdsInt *node;     // lane base, stride 64
+	LdsFloat *entry;
 };
 
 // kShort is the number of top entries held in LDS (the LDS split); kCap the
@@ -660,7 +712,7 @@ struct BvhState {
 	bool overflow;
 };
 
-template <bool kMetrics, int kCap, int kShort>
+template <bool kMetrics, int kCap, int kShort, int kSlab = kSlabAny>
 __device__ __forceinline__ void bvhBegin(const SceneRange &R, const Ray &ray, const Best &best, Counters<kMetrics> &c,
                                          BvhState<kCap, kShort> &b) {
 	b.size = 0;
@@ -672,7 +724,7 @@ __device__ __forceinline__ void bvhBegin(const SceneRange &R, const Ray &ray, co
 		if (ri < 0) { b.overflow = true; break; }
 		c.add(1, 1);
 		const float *rb = R.rootBounds[r];
-		float e = entryDistance(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ray, best.t);
+		float e = entryDistance<kSlab>(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], ray, best.t);
 		if (!isfinite(e)) continue;
 		if (b.size >= cap) { b.overflow = true; break; }
 		int pos = b.size;
@@ -700,12 +752,12 @@ __device__ __forceinline__ void bvhEnd(const DevScene &S, const SceneRange &R, c
 	}
 }
 
-template <bool kMetrics, int kCap, int kShort = kShortStack>
+template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny>
 __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
                                 Counters<kMetrics> &c, LdsStack lds) {
 	BvhState<kCap, kShort> b;
 	b.st.lds = lds;
-	bvhBegin<kMetrics>(R, ray, best, c, b);
+	bvhBegin<kMetrics, kCap, kShort, kSlab>(R, ray, best, c, b);
 	const int cap = R.stackLimit;
 	int size = b.size;
 	bool overflow = b.overflow;
@@ -735,8 +787,8 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 				} else {
 					c.add(10, 1);
 					const float4 a = rec[0], bb = rec[1], cc = rec[2];
-					const float le = entryDistance(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
-					const float re = entryDistance(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
+					const float le = entryDistance<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
+					const float re = entryDistance<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
 					c.add(1, 2);
 					int nearN = nd.x, farN = nd.y;
 					float nearE = le, farE = re;
@@ -781,8 +833,8 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 #if !MRAD_NODE_PREFETCH
 		const float4 a = rec[0], bb = rec[1], cc = rec[2];
 #endif
-		const float le = entryDistance(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
-		const float re = entryDistance(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
+		const float le = entryDistance<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
+		const float re = entryDistance<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
 		c.add(1, 2);
 		int nearN = nd.x, farN = nd.y;
 		float nearE = le, farE = re;
@@ -930,7 +982,21 @@ __device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange
 	best.kind = -1;
 	best.index = -1;
 	const Ray ray = makeRay(o, d, R.fastSlabs != 0);
+#if MRAD_SPLIT_SLAB
+	// One traversal loop per slab-quotient form, so the common (reciprocal)
+	// form's loop carries no per-test branch to the IEEE form.
+	if (R.useBvh != 0) {
+#if MRAD_SPLIT_SLAB > 1
+		if (ray.fast && d.x != 0.0f && d.y != 0.0f && d.z != 0.0f)
+			traceBvh<kMetrics, kCap, kShort, kSlabFastNz>(S, R, ray, best, c, lds);
+		else
+#endif
+		if (ray.fast) traceBvh<kMetrics, kCap, kShort, kSlabFast>(S, R, ray, best, c, lds);
+		else traceBvh<kMetrics, kCap, kShort, kSlabIeee>(S, R, ray, best, c, lds);
+	}
+#else
 	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort>(S, R, ray, best, c, lds);
+#endif
 	else traceLinear<kMetrics>(S, R, ray, best, c);
 	c.add(2, (uint32_t) R.planeCount);
 	c.add(5, (uint32_t) R.planeCount);

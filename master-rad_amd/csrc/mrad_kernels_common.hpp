@@ -22,7 +22,7 @@ using StackLds = StackLdsT<>;
 template <int kShort>
 __device__ __forceinline__ LdsStack laneStack(StackLdsT<kShort> &s) {
 	const int base = (threadIdx.x >> 6) * kShort * 64 + (threadIdx.x & 63);
-	return {s.node + base, s.entry + base};
+	return {(LdsInt *) (s.node + base), (LdsFloat *) (s.entry + base)};
 }
 
 __device__ __forceinline__ unsigned long long waveSum(unsigned long long v) {

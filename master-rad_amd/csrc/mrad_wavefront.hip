@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+
 #include <string>
 #include <vector>
 
@@ -239,11 +240,26 @@ __device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, u
 // kShadeTile entries (kShadePer per thread) in turn; the survivors of a tile
 // keep their order and take one atomic per tile on the destination count
 // (per-wave atomics on one address serialise and cost more than the shading).
-constexpr int kShadePer = 4;
+#ifndef MRAD_SHADE_PER
+#define MRAD_SHADE_PER 4
+#endif
+constexpr int kShadePer = MRAD_SHADE_PER;
 constexpr uint32_t kShadeTile = kBlock * kShadePer;
 
+#ifndef MRAD_SHADE_OCTANTS
+#define MRAD_SHADE_OCTANTS 1
+#endif
+
 __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
-	__shared__ uint32_t part[kShadePer * kWaves];
+#if MRAD_SHADE_OCTANTS
+	// Survivors of a tile are grouped by the octant of their new direction
+	// (a counting sort over 8 buckets, stable within a bucket), so a trace
+	// wave's 64 consecutive paths start near each other and head the same way.
+	constexpr int kBuckets = 8;
+#else
+	constexpr int kBuckets = 1;
+#endif
+	__shared__ uint32_t part[kBuckets * kShadePer * kWaves];
 	__shared__ uint32_t tileBase;
 	const uint32_t count = P.counters[kCount0 + cur];
 	const WfPool Q = P.pool[cur], D = P.pool[cur ^ 1];
@@ -253,17 +269,23 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 		float4 od0[kShadePer], od1[kShadePer], tl[kShadePer];
 		uint4 meta[kShadePer];
 		bool alive[kShadePer];
+		int bucket[kShadePer];
 		unsigned long long mask[kShadePer];
 #pragma unroll
 		for (int k = 0; k < kShadePer; k++) {
 			alive[k] = shadeEntry(P, Q, tile + k * kBlock + threadIdx.x, count, od0[k], od1[k], tl[k], meta[k]);
-			mask[k] = __ballot(alive[k]);
-			if (lane == 0) part[k * kWaves + wave] = (uint32_t) __popcll(mask[k]);
+			bucket[k] = (kBuckets == 1 || !alive[k]) ? 0
+			          : (od0[k].w < 0.0f ? 1 : 0) | (od1[k].x < 0.0f ? 2 : 0) | (od1[k].y < 0.0f ? 4 : 0);
+			for (int q = 0; q < kBuckets; q++) {
+				const unsigned long long m = __ballot(alive[k] && bucket[k] == q);
+				if (q == bucket[k]) mask[k] = m;
+				if (lane == 0) part[(q * kShadePer + k) * kWaves + wave] = (uint32_t) __popcll(m);
+			}
 		}
 		__syncthreads();
 		if (threadIdx.x == 0) {
 			uint32_t sum = 0;
-			for (int j = 0; j < kShadePer * kWaves; j++) {
+			for (int j = 0; j < kBuckets * kShadePer * kWaves; j++) {
 				const uint32_t n = part[j];
 				part[j] = sum;
 				sum += n;
@@ -274,7 +296,8 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 #pragma unroll
 		for (int k = 0; k < kShadePer; k++) {
 			if (alive[k]) {
-				const uint32_t slot = tileBase + part[k * kWaves + wave] + (uint32_t) __popcll(mask[k] & below);
+				const uint32_t slot = tileBase + part[(bucket[k] * kShadePer + k) * kWaves + wave] +
+				                      (uint32_t) __popcll(mask[k] & below);
 				D.od0[slot] = od0[k];
 				D.od1[slot] = od1[k];
 				D.tl[slot] = tl[k];
@@ -285,8 +308,6 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 	}
 }
 
-// Per-pixel sum of its samples in sample order with the megakernel's last-
-// sample normalization (GpuKernelSources.java:1284-1293).
 __global__ __launch_bounds__(kBlock) void wfResolve(WfParams P) {
 	for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < P.totalPixels; p += gridDim.x * kBlock) {
 		const WfScene &V = P.scenes[sceneOfPixel(P, p)];
