@@ -226,7 +226,7 @@ typedef struct {
 	int collect_metrics;     /* counter variant of the trace kernel */
 	int use_graph;           /* capture the bounce steps as a hipGraph */
 	int time_kernels;        /* time each kernel with HIP events (no graph; for measurement) */
-	int lds_stack_entries;   /* traversal-stack entries per lane held in LDS: 4 or 8 (0 = 8) */
+	int lds_stack_entries;   /* traversal-stack entries per lane held in LDS: 4 or 10 (0 = 10) */
 	int leaf_regroup;        /* 1 = regroup each wave's leaf tests by primitive class */
 } mrad_wavefront_config;
 

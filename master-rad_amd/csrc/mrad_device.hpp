@@ -535,6 +535,44 @@ __device__ __forceinline__ float entryDistanceFastNz(float lx, float ly, float l
 #undef MRAD_ENTRY_AXIS
 	return tMin <= cb ? tMin : INFINITY;
 }
+// Branch-free forms of the two reciprocal entry distances: every axis is
+// computed and the slab test is made once on the final interval.  The result
+// is the same: the interval only shrinks, so it ends empty exactly when one
+// of the per-axis checks would have returned; a zero direction component
+// contributes no bounds, or an empty interval when the origin lies outside
+// the slab (the reference's zero-direction branch).
+__device__ __forceinline__ void slabAxisBf(float oc, float dc, float inv, float lo, float hi, bool nz, float &tMin,
+                                           float &tMax) {
+	const float n0 = lo - oc, n1 = hi - oc;
+	const float q0 = n0 * inv, q1 = n1 * inv;
+	float t0 = fmaf(fmaf(-q0, dc, n0), inv, q0);
+	float t1 = fmaf(fmaf(-q1, dc, n1), inv, q1);
+	const float a = fminf(t0, t1), b0 = fmaxf(t0, t1);
+	const float b = b0 > 0.0f ? b0 * 1.00000072f : b0;
+	if (nz) {
+		tMin = fmaxf(tMin, a);
+		tMax = fminf(tMax, b);
+	} else {
+		const bool zero = dc == 0.0f;
+		const bool outside = oc < lo || oc > hi;
+		tMin = fmaxf(tMin, zero ? (outside ? INFINITY : -INFINITY) : a);
+		tMax = fminf(tMax, zero ? INFINITY : b);
+	}
+}
+template <bool kNz>
+__device__ __forceinline__ float entryDistanceFastBf(float lx, float ly, float lz, float hx, float hy, float hz,
+                                                     const Ray &r, float beforeT) {
+	float tMin = kEps;
+	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
+	float tMax = cb;
+	slabAxisBf(r.o.x, r.d.x, r.inv.x, lx, hx, kNz, tMin, tMax);
+	slabAxisBf(r.o.y, r.d.y, r.inv.y, ly, hy, kNz, tMin, tMax);
+	slabAxisBf(r.o.z, r.d.z, r.inv.z, lz, hz, kNz, tMin, tMax);
+	return (tMax < tMin || !(tMin <= cb)) ? INFINITY : tMin;
+}
+#ifndef MRAD_ENTRY_BRANCH_FREE
+#define MRAD_ENTRY_BRANCH_FREE 1
+#endif
 // bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.  kSlab picks
 // the quotient form: kSlabAny by the ray's flag, kSlabFast or kSlabIeee when
 // the caller already knows it (a traversal loop specialised per form keeps
@@ -543,6 +581,10 @@ constexpr int kSlabAny = 0, kSlabFast = 1, kSlabIeee = 2, kSlabFastNz = 3;
 template <int kSlab = kSlabAny>
 __device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, const Ray &r,
                                                float beforeT) {
+#if MRAD_ENTRY_BRANCH_FREE
+	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastBf<true>(lx, ly, lz, hx, hy, hz, r, beforeT);
+	if constexpr (kSlab == kSlabFast) return entryDistanceFastBf<false>(lx, ly, lz, hx, hy, hz, r, beforeT);
+#endif
 	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastNz(lx, ly, lz, hx, hy, hz, r, beforeT);
 	if constexpr (kSlab == kSlabFast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, beforeT);
 	if constexpr (kSlab == kSlabIeee) return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, beforeT);
@@ -650,14 +692,14 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 // 1 specialises the traversal loop per slab-quotient form (see traceClosest);
 // 2 adds a loop for reciprocal-form rays with no zero direction component.
 #ifndef MRAD_SPLIT_SLAB
-#define MRAD_SPLIT_SLAB 1
+#define MRAD_SPLIT_SLAB 2
 #endif
 // 1 fetches a popped node's whole 64-byte record before its kind is known.
 #ifndef MRAD_NODE_PREFETCH
 #define MRAD_NODE_PREFETCH 0
 #endif
 #ifndef MRAD_SHORT_STACK
-#define MRAD_SHORT_STACK 8
+#define MRAD_SHORT_STACK 10
 #endif
 constexpr int kShortStack = MRAD_SHORT_STACK;
 

@@ -6,13 +6,6 @@
 // compilation.
 #include <hip/hip_runtime.h>
 
-// The megakernel and replay kernels add a traversal loop for rays with no
-// zero direction component (3.7 % faster megakernel on the same GPU; the
-// wavefront trace kernel, measured the same way, is faster without it).
-#ifndef MRAD_SPLIT_SLAB
-#define MRAD_SPLIT_SLAB 2
-#endif
-
 #include <algorithm>
 #include <cstring>
 #include <mutex>

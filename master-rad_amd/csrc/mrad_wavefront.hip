@@ -594,7 +594,7 @@ template <bool M>
 TraceFn traceFor(int cap, int ldsEntries, bool regroup) {
 	if (regroup) return cap <= 64 ? wfTraceRegroup<M, 64, 8> : wfTraceRegroup<M, 256, 8>;
 	if (ldsEntries <= 4) return cap <= 64 ? wfTrace<M, 64, 4> : wfTrace<M, 256, 4>;
-	return cap <= 64 ? wfTrace<M, 64, 8> : wfTrace<M, 256, 8>;
+	return cap <= 64 ? wfTrace<M, 64, kShortStack> : wfTrace<M, 256, kShortStack>;
 }
 
 }  // namespace mrad
@@ -749,9 +749,10 @@ int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out) {
 	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
 	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
 	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
-	if (c.lds_stack_entries == 0) c.lds_stack_entries = 8;
-	if (c.lds_stack_entries != 4 && c.lds_stack_entries != 8)
-		return set_error(MRAD_ERR_INVALID_ARGUMENT, "LDS stack entries must be 4 or 8");
+	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
+	if (c.lds_stack_entries != 4 && c.lds_stack_entries != kShortStack)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT,
+		                 "LDS stack entries must be 4 or " + std::to_string(kShortStack));
 	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28))
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "queue capacity must be in [64, 2^28]");
 	if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return set_error(MRAD_ERR_INVALID_ARGUMENT, "waves per CU must be in [1, 32]");
@@ -912,9 +913,9 @@ int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
 	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
 	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
 	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
-	if (c.lds_stack_entries == 0) c.lds_stack_entries = 8;
+	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
 	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28) || c.waves_per_cu < 1 || c.waves_per_cu > 32 ||
-	    (c.lds_stack_entries != 4 && c.lds_stack_entries != 8))
+	    (c.lds_stack_entries != 4 && c.lds_stack_entries != kShortStack))
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "invalid wavefront configuration");
 	std::lock_guard<std::mutex> g(b->mu);
 	if (c.device != b->cfg.device || c.bvh_stack_size != b->cfg.bvh_stack_size || c.linear_scan != b->cfg.linear_scan)

@@ -25,7 +25,7 @@ class WavefrontConfig:
     collect_metrics: bool = False
     use_graph: bool = True
     time_kernels: bool = False
-    lds_stack_entries: int = 8
+    lds_stack_entries: int = 10
     leaf_regroup: bool = False
     device: int = 0
 

@@ -61,7 +61,7 @@ def test_batch_counters_equal_megakernel(require_gpu):
         assert st[k] == ref_stats[k], k
 
 
-@pytest.mark.parametrize("regroup,lds", [(True, 8), (False, 4)])
+@pytest.mark.parametrize("regroup,lds", [(True, 10), (False, 4)])
 def test_regroup_and_lds_split_bit_exact_with_counters(require_gpu, regroup, lds):
     """Primitive-class regrouping of leaf tests and the 4-entry LDS split change neither images nor counters."""
     cases = [(lambda: small_field(4000, 5, clustered=True), configs.field_camera, 80, 45, 3, 6, -1),
@@ -91,7 +91,7 @@ def test_configure_keeps_scenes_and_images(require_gpu):
         b.render()
         first = b.image(0)
         best, table = tune(b, grid={"queue_capacity": [1024, 8192], "waves_per_cu": [4, 8],
-                                    "lds_stack_entries": [4, 8]})
+                                    "lds_stack_entries": [4, 10]})
         assert len(table) >= 4
         b.render()
         np.testing.assert_array_equal(first, b.image(0))
