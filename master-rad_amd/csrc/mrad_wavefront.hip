@@ -18,6 +18,13 @@
 // both forms produce bit-identical images.
 #include <hip/hip_runtime.h>
 
+// The trace kernel is compiled to the 72-VGPR budget of 7 waves per SIMD;
+// carrying the nearer child in registers costs it 3.3 % there (and 7 % at 6
+// waves per SIMD), so its traversal pushes both children.
+#ifndef MRAD_CARRY_NEAR
+#define MRAD_CARRY_NEAR 0
+#endif
+
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -160,7 +167,7 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 // Occupancy the trace kernels are compiled for (waves per SIMD; the register
 // budget is 512 / this).
 #ifndef MRAD_TRACE_WAVES_PER_EU
-#define MRAD_TRACE_WAVES_PER_EU 7
+#define MRAD_TRACE_WAVES_PER_EU 8
 #endif
 
 // Closest hit of every live path in pool `cur` (persistent).
@@ -312,12 +319,27 @@ __global__ __launch_bounds__(kBlock) void wfResolve(WfParams P) {
 	for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < P.totalPixels; p += gridDim.x * kBlock) {
 		const WfScene &V = P.scenes[sceneOfPixel(P, p)];
 		const uint32_t pixel = p - V.pixelBase;
-		const float *r = P.results + 3 * ((size_t) V.pathBase + (size_t) pixel * V.spp);
+		const size_t first = (size_t) V.pathBase + (size_t) pixel * V.spp;
+		const float *r = P.results + 3 * first;
 		f3 acc = mk(0.0f, 0.0f, 0.0f);
-		for (int s = 0; s < V.spp; s++) {
-			const f3 col = mk(r[3 * s], r[3 * s + 1], r[3 * s + 2]);
-			acc = s == 0 ? col : acc + col;
-			if (s == V.spp - 1) acc = acc * (1.0f / (float) V.spp);
+		if ((V.spp & 3) == 0 && (first & 3) == 0) {
+			// four samples (12 floats) per three 16-byte loads, summed in sample order
+			const float4 *q = reinterpret_cast<const float4 *>(r);
+			for (int s = 0; s < V.spp; s += 4) {
+				const float4 a = q[0], b = q[1], c = q[2];
+				q += 3;
+				acc = s == 0 ? mk(a.x, a.y, a.z) : acc + mk(a.x, a.y, a.z);
+				acc = acc + mk(a.w, b.x, b.y);
+				acc = acc + mk(b.z, b.w, c.x);
+				acc = acc + mk(c.y, c.z, c.w);
+			}
+			acc = acc * (1.0f / (float) V.spp);
+		} else {
+			for (int s = 0; s < V.spp; s++) {
+				const f3 col = mk(r[3 * s], r[3 * s + 1], r[3 * s + 2]);
+				acc = s == 0 ? col : acc + col;
+				if (s == V.spp - 1) acc = acc * (1.0f / (float) V.spp);
+			}
 		}
 		P.images[3 * (size_t) p] = acc.x;
 		P.images[3 * (size_t) p + 1] = acc.y;
@@ -747,7 +769,7 @@ int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out) {
 	*out = nullptr;
 	mrad_wavefront_config c = *config;
 	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
-	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 32;
 	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
 	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
 	if (c.lds_stack_entries != 4 && c.lds_stack_entries != kShortStack)
@@ -911,7 +933,7 @@ int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
 	if (!config) return set_error(MRAD_ERR_INVALID_ARGUMENT, "config is required");
 	mrad_wavefront_config c = *config;
 	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
-	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 32;
 	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
 	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
 	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28) || c.waves_per_cu < 1 || c.waves_per_cu > 32 ||

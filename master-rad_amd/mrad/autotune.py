@@ -16,7 +16,7 @@ from .batch import BatchRenderer, WavefrontConfig
 
 GRID = {
     "queue_capacity": [1 << 21, 1 << 22, 1 << 23, 1 << 24],
-    "waves_per_cu": [16, 24, 28],
+    "waves_per_cu": [16, 24, 28, 32],
     "lds_stack_entries": [4, 10],
 }
 
