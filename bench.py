@@ -246,11 +246,27 @@ def bench_batch(args) -> dict:
         "segments_per_path": st_m["rays"] / total_paths,
         "adjoint": adj,
         "adjoint_over_forward": adj["ratio"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("batch-64"),
-                     "kernel": "wfTrace", "kernel_ms_per_step": trace_s * 1e3},
+        "roofline": roofline_entry(achieved, load_traffic("batch-64"), trace_s / max(1, st_t["steps"]), "wfTrace",
+                                   trace_s * 1e3, st_t["steps"]),
         "counters_per_step": {k: st_m[k] for k in mrad.STAT_NAMES},
     }, ws
+
+
+def roofline_entry(achieved: float, traffic, launch_s: float, kernel: str, kernel_ms_per_step: float,
+                   launches: int) -> dict:
+    """The roofline object of the dominant kernel.  `achieved` prices every
+    node and primitive fetch of the exact counters as if it came from HBM, so
+    it can exceed the HBM peak when the caches serve most fetches;
+    `traffic_gbs` is the measured HBM traffic of a launch (rocprofv3
+    FETCH_SIZE + WRITE_SIZE, profiles/traffic_*.json) over the measured
+    average launch time, the bandwidth the kernel actually draws."""
+    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+         "kernel_ms_per_step": kernel_ms_per_step, "launches_per_step": launches}
+    if traffic is not None and launch_s > 0:
+        r["traffic_gbs"] = traffic / launch_s / 1e9
+        r["traffic_frac"] = r["traffic_gbs"] / HBM_PEAK_GBS
+    return r
 
 
 def bench_adjoint() -> dict:
@@ -424,7 +440,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batch", "het", "clustered", "hero"], default="batch")
-    ap.add_argument("--queue-capacity", type=int, default=1 << 23)
+    ap.add_argument("--queue-capacity", type=int, default=1 << 26)
     ap.add_argument("--waves-per-cu", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",

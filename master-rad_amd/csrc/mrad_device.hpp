@@ -227,6 +227,14 @@ __device__ __forceinline__ Ray makeRay(f3 o, f3 d, bool sceneFast) {
 	         origOk(o.x) && origOk(o.y) && origOk(o.z);
 	return r;
 }
+// Slab-quotient form of a call site: kSlabAny reads the ray's flag, the
+// others are known at compile time (a traversal loop specialised per form).
+constexpr int kSlabAny = 0, kSlabFast = 1, kSlabIeee = 2, kSlabFastNz = 3;
+template <int kSlab>
+__device__ __forceinline__ bool rayFast(const Ray &r) {
+	if constexpr (kSlab == kSlabAny) return r.fast;
+	else return kSlab != kSlabIeee;
+}
 // RN(num / den) from inv = RN(1 / den); num == 0 or |num| in [2^-60, 2^60].
 __device__ __forceinline__ float exactQuotient(float num, float den, float inv) {
 	const float q = num * inv;
@@ -277,9 +285,10 @@ __device__ __forceinline__ bool sphereSolveT(float4 s, const Ray &ray, SphereSol
 	r.closest = cl;
 	return true;
 }
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ bool sphereSolve(float4 s, const Ray &ray, SphereSolve &r) {
 	bool ok = true;
-	if (ray.fast) {
+	if (rayFast<kSlab>(ray)) {
 		SphereSolve f;
 		const bool hit = sphereSolveT<true>(s, ray, f, ok);
 		if (ok) {
@@ -289,9 +298,10 @@ __device__ __forceinline__ bool sphereSolve(float4 s, const Ray &ray, SphereSolv
 	}
 	return sphereSolveT<false>(s, ray, r, ok);
 }
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ bool testSphere(float4 s, const Ray &ray, float limit, float &t) {
 	SphereSolve r;
-	if (!sphereSolve(s, ray, r)) return false;
+	if (!sphereSolve<kSlab>(s, ray, r)) return false;
 	t = r.t;
 	return r.t < limit;
 }
@@ -348,10 +358,11 @@ __device__ __forceinline__ f3 axisNormal(int axis, float sign) {
 }
 
 // traceBox body (GpuKernelSources.java:594-684).
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ bool boxSolve(float4 b0, float4 b1, const Ray &ray, BoxSolve &r) {
 	float x0, x1, y0, y1, z0, z1;
 	const f3 o = ray.o, d = ray.d;
-	if (ray.fast) {
+	if (rayFast<kSlab>(ray)) {
 		if (!slabFast(o.x, d.x, ray.inv.x, b0.x, b0.w, x0, x1)) return false;
 		if (!slabFast(o.y, d.y, ray.inv.y, b0.y, b1.x, y0, y1)) return false;
 		if (!slabFast(o.z, d.z, ray.inv.z, b0.z, b1.y, z0, z1)) return false;
@@ -362,9 +373,10 @@ __device__ __forceinline__ bool boxSolve(float4 b0, float4 b1, const Ray &ray, B
 	}
 	return boxSelect(x0, x1, y0, y1, z0, z1, kEps, d, r);
 }
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ bool testBox(float4 b0, float4 b1, const Ray &ray, float limit, float &t) {
 	BoxSolve r;
-	if (!boxSolve(b0, b1, ray, r)) return false;
+	if (!boxSolve<kSlab>(b0, b1, ray, r)) return false;
 	t = r.t;
 	return !(r.t >= limit);
 }
@@ -372,11 +384,12 @@ __device__ __forceinline__ bool testBox(float4 b0, float4 b1, const Ray &ray, fl
 // affineRayShift / applyAffine* (GpuKernelSources.java:276-301); rows hold
 // the inverse linear map with the world centre in .w.
 struct AffineLocal { float shift; f3 lo, ld; };
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ void affineLocal(float4 r0, float4 r1, float4 r2, const Ray &ray, AffineLocal &a) {
 	const f3 o = ray.o, d = ray.d;
 	f3 c = mk(r0.w, r1.w, r2.w);
 	float den = ray.dd;
-	a.shift = den > 0.0f ? fmaxf(0.0f, quotient(dot(c - o, d), den, ray.invDD, ray.fast)) : 0.0f;
+	a.shift = den > 0.0f ? fmaxf(0.0f, quotient(dot(c - o, d), den, ray.invDD, rayFast<kSlab>(ray))) : 0.0f;
 	f3 rel = mk(fmaf(d.x, a.shift, o.x - r0.w), fmaf(d.y, a.shift, o.y - r1.w), fmaf(d.z, a.shift, o.z - r2.w));
 	a.lo = mk(r0.x * rel.x + r0.y * rel.y + r0.z * rel.z,
 	          r1.x * rel.x + r1.y * rel.y + r1.z * rel.z,
@@ -429,9 +442,10 @@ __device__ __forceinline__ bool affineSphereSolve(const AffineLocal &a, float &h
 	worldT = a.shift + hitT;
 	return true;
 }
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ bool testAffineSphere(const float4 *rec, const Ray &ray, float limit, float &t) {
 	AffineLocal a;
-	affineLocal(rec[0], rec[1], rec[2], ray, a);
+	affineLocal<kSlab>(rec[0], rec[1], rec[2], ray, a);
 	float hitT, worldT;
 	if (!affineSphereSolve(a, hitT, worldT)) return false;
 	t = worldT;
@@ -448,9 +462,10 @@ __device__ __forceinline__ bool affineBoxSolve(const AffineLocal &a, BoxSolve &r
 	worldT = a.shift + r.t;
 	return true;
 }
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ bool testAffineBox(const float4 *rec, const Ray &ray, float limit, float &t) {
 	AffineLocal a;
-	affineLocal(rec[0], rec[1], rec[2], ray, a);
+	affineLocal<kSlab>(rec[0], rec[1], rec[2], ray, a);
 	BoxSolve r;
 	float worldT;
 	if (!affineBoxSolve(a, r, worldT)) return false;
@@ -577,7 +592,6 @@ __device__ __forceinline__ float entryDistanceFastBf(float lx, float ly, float l
 // the quotient form: kSlabAny by the ray's flag, kSlabFast or kSlabIeee when
 // the caller already knows it (a traversal loop specialised per form keeps
 // one form on its hot path).
-constexpr int kSlabAny = 0, kSlabFast = 1, kSlabIeee = 2, kSlabFastNz = 3;
 template <int kSlab = kSlabAny>
 __device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, const Ray &r,
                                                float beforeT) {
@@ -621,6 +635,7 @@ __device__ __forceinline__ int typeSlot(uint32_t type) {
 
 // tracePrimitiveRef with the reference's (t, order) acceptance
 // (GpuKernelSources.java:772-806) on the primitive's test record `rec`.
+template <int kSlab = kSlabAny>
 __device__ __forceinline__ void testRefAt(const float4 *rec, uint32_t ref, uint32_t type, const Ray &ray, Best &best) {
 	const int index = (int) (ref & kRefIndexMask);
 	const float limit = nextafterf(best.t, INFINITY);
@@ -629,16 +644,16 @@ __device__ __forceinline__ void testRefAt(const float4 *rec, uint32_t ref, uint3
 	int kind;
 	if (type == kRefBox) {
 		kind = kKindBox;
-		found = testBox(rec[0], rec[1], ray, limit, t);
+		found = testBox<kSlab>(rec[0], rec[1], ray, limit, t);
 	} else if (type == kRefAffineSphere) {
 		kind = kKindAffineSphere;
-		found = testAffineSphere(rec, ray, limit, t);
+		found = testAffineSphere<kSlab>(rec, ray, limit, t);
 	} else if (type == kRefAffineBox) {
 		kind = kKindAffineBox;
-		found = testAffineBox(rec, ray, limit, t);
+		found = testAffineBox<kSlab>(rec, ray, limit, t);
 	} else {
 		kind = kKindSphere;
-		found = testSphere(rec[0], ray, limit, t);
+		found = testSphere<kSlab>(rec[0], ray, limit, t);
 	}
 	const uint32_t rank = type | (uint32_t) index;
 	if (!found || t > best.t || (t == best.t && rank >= best.rank)) return;
@@ -656,7 +671,7 @@ __device__ __forceinline__ void testRef(const DevScene &S, uint32_t ref, uint32_
 // The refs of a leaf node record `rec` with data `nd` = {kind, ref if one,
 // first ref, count}.  A one-ref leaf carries its primitive's test record in
 // the node record (rec[0..2]), so only the node itself is fetched.
-template <bool kMetrics>
+template <bool kMetrics, int kSlab = kSlabAny>
 __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R, const float4 *rec, int4 nd,
                                          const Ray &ray, Best &best, Counters<kMetrics> &c) {
 	c.add(11, 1);
@@ -673,7 +688,7 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 		const uint32_t ref = inlined ? (uint32_t) nd.y : S.refs[ri];
 		const uint32_t type = mixed ? (ref & kRefTypeMask) : leafType;
 		c.add(3 + typeSlot(type), 1);
-		testRefAt(inlined ? rec : primitiveRecord(S, ref, type), ref, type, ray, best);
+		testRefAt<kSlab>(inlined ? rec : primitiveRecord(S, ref, type), ref, type, ray, best);
 	}
 }
 
@@ -854,7 +869,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		const unsigned long long waiting = __ballot(pending);
 		if (waiting == __ballot(1) || __popcll(waiting) >= MRAD_LEAF_BATCH) {
 			if (pending) {
-				testLeaf<kMetrics>(S, R, S.nodes + 4 * (size_t) leafNode, leaf, ray, best, c);
+				testLeaf<kMetrics, kSlab>(S, R, S.nodes + 4 * (size_t) leafNode, leaf, ray, best, c);
 				pending = false;
 			}
 		}
@@ -883,7 +898,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		const float4 *rec = S.nodes + 4 * (size_t) node;
 		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
 		if (nd.x < 0) {
-			testLeaf<kMetrics>(S, R, rec, nd, ray, best, c);
+			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
 			continue;
 		}
 		c.add(10, 1);
@@ -920,7 +935,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 #endif
 		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
 		if (nd.x < 0) {
-			testLeaf<kMetrics>(S, R, rec, nd, ray, best, c);
+			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
 			continue;
 		}
 		c.add(10, 1);

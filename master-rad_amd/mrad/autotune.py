@@ -15,7 +15,7 @@ import time
 from .batch import BatchRenderer, WavefrontConfig
 
 GRID = {
-    "queue_capacity": [1 << 21, 1 << 22, 1 << 23, 1 << 24],
+    "queue_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26],
     "waves_per_cu": [16, 24, 28, 32],
     "lds_stack_entries": [4, 10],
 }

@@ -37,7 +37,7 @@ def _batch():
     from mrad import configs
     ws = configs.batch(64, threads=16)
     print("batch built", flush=True)
-    b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 23))
+    b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 26))
     for w in ws:
         b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, 0, w.frame_seed, w.rr_start)
     st = b.render()
