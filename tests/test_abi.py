@@ -91,3 +91,17 @@ def test_conditioning_stats_validation():
                 (1, 0, 100, 101, 500), (1, 0, 100, 60, 0)]:
         with pytest.raises(mrad.InvalidArgument):
             mrad.ConditioningStats(*bad)
+
+
+def test_calibration_entry_points_check_arguments_first():
+    """mrad_cost_create / mrad_cost_time reject bad arguments before touching a device."""
+    h = C.c_void_p()
+    cus = C.c_int()
+    w = np.zeros(16, np.float32)
+    for records, profiles in ((100, 3), (0, 3), (64, 0)):
+        rc = N.lib.mrad_cost_create(0, w.ctypes.data_as(N.fp), records, profiles, C.byref(cus), C.byref(h))
+        assert rc == N.MRAD_ERR_INVALID_ARGUMENT
+    assert N.lib.mrad_cost_create(0, None, 64, 3, C.byref(cus), C.byref(h)) == N.MRAD_ERR_INVALID_ARGUMENT
+    ms = C.c_float()
+    assert N.lib.mrad_cost_time(None, 1, 0, 0, 1, 1, 1, 64, C.byref(ms)) == N.MRAD_ERR_INVALID_ARGUMENT
+    N.lib.mrad_cost_destroy(None)
