@@ -21,8 +21,8 @@ every scene resident in HBM and every image left in HBM.  Reported beside it:
 clustered (1M clustered primitives, half affine, 1080p, 16 spp, 6 bounces) time
 one full-size frame in the megakernel, the wavefront and the regrouped
 wavefront forms.  --workload hero times the reference's own Layered Hero
-frames (96 to 1M primitives, 1080p, 1 spp, 12 bounces) under the uniform and
-weighted SAH.
+frames (96 to 1M primitives, 1080p, 1 spp, 12 bounces) and its overlap-10k
+gallery frame under the uniform and weighted SAH.
 """
 from __future__ import annotations
 
@@ -416,6 +416,26 @@ def bench_hero(args) -> dict:
                      "published_rtx3080_time_reduction_pct": pub[(rid, "weighted")]["timeReductionPercent"]})
         if scale == W.HERO_SCALES[-1]:
             line = dict(res["weighted"], scene=sc)
+    # the overlap-10k gallery frame (GalleryOverlapScene), uniform and weighted
+    ovl = {(r["rowId"], r["method"]): r for r in json.loads((ROOT / "tests" / "golden" / "overlap_10k.json").read_text())}
+    ocam = W.gpu_camera(W.overlap_camera(), W.WIDTH, W.HEIGHT)
+    okey = W.workload_key("overlap-10k")
+    oms = {}
+    for method, lam in (("uniform", 0.0), ("weighted", 1.0)):
+        cfg = mrad.BvhBuildConfig.uniform(W.LEAF_SIZE) if lam == 0.0 else mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, lam)
+        sc = W.overlap_scene(cfg)
+        with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
+            t.render_sample(None, sc, ocam, W.MAXIMUM_PATH_DEPTH, 0, W.timing_seed(okey, 0))
+            kern = []
+            for i in range(max(3, frames // 4)):
+                t.render_sample(None, sc, ocam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(W.timing_seed(okey, 0), i))
+                kern.append(t.last_frame_stats()["kernel_nanos"])
+        oms[method] = float(np.median(kern)) / 1e6
+        log(f"overlap-10k {method}: {oms[method]:.2f} ms kernel")
+    rows.append({"row": "overlap-10k", "primitives": W.OVERLAP_OBJECTS, "uniform_kernel_ms": oms["uniform"],
+                 "weighted_kernel_ms": oms["weighted"],
+                 "weighted_time_reduction_pct": 100.0 * (1.0 - oms["weighted"] / oms["uniform"]),
+                 "published_rtx3080_time_reduction_pct": ovl[("overlap-10k", "weighted")]["timeReductionPercent"]})
     paths = W.WIDTH * W.HEIGHT
     return {
         "metric": METRIC, "value": paths / (line["wall_ms"] / 1e3) / 1e6, "unit": "Mpaths/s", "n_gpus": 1,

@@ -362,3 +362,136 @@ def hero_gpu_camera(width: int = WIDTH, height: int = HEIGHT) -> GpuCamera:
 def scale_id(scale: int) -> str:
     """BenchmarkProtocol.compact (:410-417) as used in the row ids."""
     return "hero-" + {1_000: "1k", 10_000: "10k", 100_000: "100k", 1_000_000: "1m"}.get(scale, str(scale))
+
+
+# ---- Gallery overlap (graphics3d/scene/v1/GalleryOverlapSource.java, GalleryOverlapScene.java)
+OVERLAP_OBJECTS = 10_000
+_OVERLAP_SPHERES, _OVERLAP_LARGE, _OVERLAP_FOREGROUND = 3_000, 1_000, 128
+_GOLDEN_ANGLE_TURNS = 0.3819660112501051
+
+
+def _unit_hash(index: int, salt: int) -> float:
+    """GalleryOverlapSource.unit / GalleryOverlapScene.unit (Java long arithmetic)."""
+    v = (index * 0x9E3779B97F4A7C15 + salt) & _M64
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & _M64
+    v ^= v >> 31
+    return (v >> 11) * 2.0 ** -53
+
+
+def _evenly(index: int, population: int, selected: int) -> bool:
+    return (index + 1) * selected // population != index * selected // population
+
+
+_OVL_MATS = [_surface((0.46, 0.018, 0.006)), _surface((0.006, 0.17, 0.62)), _surface((0.014, 0.42, 0.22)),
+             _surface((0.70, 0.16, 0.010))]
+_OVL_SPHERE_MATS = [_surface((0.10, 0.56, 0.88)), _surface((0.92, 0.23, 0.035)), _surface((0.055, 0.72, 0.34)),
+                    _surface((0.90, 0.50, 0.040))]
+_OVL_MIRROR = _surface((0.0, 0.0, 0.0), reflective=(0.92, 0.92, 0.92), ior=1.5)
+_OVL_GLASS = _surface((0.0, 0.0, 0.0), reflective=(0.05, 0.05, 0.05), refractive=(0.95, 0.95, 0.95), ior=1.45)
+
+
+def _v_sub(a, b): return (a[0] - b[0], a[1] - b[1], a[2] - b[2])
+def _v_add(a, b): return (a[0] + b[0], a[1] + b[1], a[2] + b[2])
+def _v_mul(a, k): return (a[0] * k, a[1] * k, a[2] * k)
+def _v_dot(a, b): return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+def _v_len(a): return math.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+def _v_cross(a, b): return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+def _v_rejection(a, d): return _v_sub(a, _v_mul(d, _v_dot(a, d) / _v_dot(d, d)))
+def _v_normalized_to(a, length): return _v_mul(a, length / _v_len(a))
+
+
+def overlap_primitives():
+    """GalleryOverlapScene.create (:47-126) over GalleryOverlapSource.overlapGallery
+    (:62-151): 3,000 spheres and 7,000 thin ribbons (affine boxes) in the
+    reference's insertion order."""
+    spheres, sphere_mats, aboxes, abox_mats = [], [], [], []
+    sphere_n = large_n = fg_n = inter_n = 0
+    for index in range(OVERLAP_OBJECTS):
+        turn = index * _GOLDEN_ANGLE_TURNS
+        turn = turn - math.floor(turn)
+        angle = turn * 2.0 * math.pi
+        progress = (index + 0.5) / OVERLAP_OBJECTS
+        radius = 0.12 + 1.05 * math.sqrt(progress)
+        phase = angle + 5.0 * math.pi * progress
+        sx, sy = 1.34 * radius * math.cos(phase), 0.72 * radius * math.sin(phase)
+        rot_turns = turn + 0.25 + 0.020 * math.sin(phase * 3.0)
+        sphere = _evenly(index, OVERLAP_OBJECTS, _OVERLAP_SPHERES)
+        large = sphere and _evenly(sphere_n, _OVERLAP_SPHERES, _OVERLAP_LARGE)
+        fg = large and _evenly(large_n, _OVERLAP_LARGE, _OVERLAP_FOREGROUND)
+        depth = 5.03 + 0.07 * _unit_hash(index, 0x4B1D5A77) if fg else 5.2 + 7.8 * _unit_hash(index, 0x4B1D5A77)
+        mat = _OVL_MATS[index & 3]
+        if sphere:
+            u = _unit_hash(index, 0x23A4D159)
+            r_sensor = (0.018 + 0.006 * u if fg else 0.013 + 0.005 * u) if large else 0.0040 + 0.0020 * u
+            if large:
+                if fg:
+                    cls = (fg_n * 53) % 128
+                    fg_n += 1
+                    mat = _OVL_GLASS if cls < 48 else _OVL_MIRROR if cls < 96 else _OVL_SPHERE_MATS[large_n & 3]
+                else:
+                    cls = (inter_n * 361) % 872
+                    inter_n += 1
+                    mat = _OVL_GLASS if cls < 80 else _OVL_MIRROR if cls < 160 else _OVL_SPHERE_MATS[large_n & 3]
+                large_n += 1
+            spheres.append((*_world(sx, sy, depth), r_sensor * depth / _PERSPECTIVE_Z))
+            sphere_mats.append(mat)
+            sphere_n += 1
+            continue
+        # the source's thin axis-aligned box (GalleryOverlapSource :104-118) ...
+        half_len_s = 1.03 + 0.34 * _unit_hash(index, 0x6F23E991)
+        half_w_s = 0.000045 + 0.000025 * _unit_hash(index, 0x19C5B7D3)
+        radians = rot_turns * 2.0 * math.pi
+        horizontal = abs(math.sin(radians)) >= abs(math.cos(radians))
+        hl, hw = half_len_s * depth / _PERSPECTIVE_Z, half_w_s * depth / _PERSPECTIVE_Z
+        half = (hl, hw, 0.008) if horizontal else (hw, hl, 0.008)
+        c0 = _world(sx, sy, depth)
+        p, q = _v_sub(c0, half), _v_add(c0, half)
+        # ... turned into a ribbon across depth (GalleryOverlapScene :69-110)
+        centre = _v_mul(_v_add(p, q), 0.5)
+        radii = _v_mul(_v_sub(q, p), 0.5)
+        horiz = radii[0] > radii[1]
+        old_len = radii[0] if horiz else radii[1]
+        old_w = radii[1] if horiz else radii[0]
+        scx = _PERSPECTIVE_Z * centre[0] / centre[2]
+        scy = _PERSPECTIVE_Z * centre[1] / centre[2]
+        s_len = _PERSPECTIVE_Z * old_len / centre[2]
+        s_w = _PERSPECTIVE_Z * old_w / centre[2]
+        near = 5.04 + 0.14 * _unit_hash(index, 0x754A32D192ED03)
+        far = 12.82 + 0.14 * _unit_hash(index, 0x16B4CE29890A1F)
+        reverse = _unit_hash(index, 0x6D2B79F5AA010D) < 0.5
+        d_minus, d_plus = (far, near) if reverse else (near, far)
+        m_s = (scx - s_len, scy, 0.0) if horiz else (scx, scy - s_len, 0.0)
+        p_s = (scx + s_len, scy, 0.0) if horiz else (scx, scy + s_len, 0.0)
+        m_e, p_e = _world(m_s[0], m_s[1], d_minus), _world(p_s[0], p_s[1], d_plus)
+        r_centre = _v_mul(_v_add(m_e, p_e), 0.5)
+        long_axis = _v_mul(_v_sub(p_e, m_e), 0.5)
+        mean_depth = 0.5 * (near + far)
+        new_w = s_w * mean_depth / _PERSPECTIVE_Z
+        want = (0.0, 1.0, 0.0) if horiz else (-1.0, -0.0, -0.0)
+        w_axis = _v_normalized_to(_v_rejection(want, long_axis), new_w)
+        t_axis = _v_normalized_to(_v_cross(long_axis, w_axis), 0.008)
+        # Affine.unitVectors(long, width, thickness).then(translation(centre))
+        t = (long_axis[0], w_axis[0], t_axis[0], 0.0, long_axis[1], w_axis[1], t_axis[1], 0.0,
+             long_axis[2], w_axis[2], t_axis[2], 0.0)
+        aboxes.append(then(t, translation(*r_centre)))
+        abox_mats.append(mat)
+    if (sphere_n, large_n, fg_n, len(aboxes)) != (_OVERLAP_SPHERES, _OVERLAP_LARGE, _OVERLAP_FOREGROUND, 7_000):
+        raise RuntimeError("overlap population mismatch")
+    return {"spheres": (np.array(spheres, np.float64), np.array(sphere_mats)),
+            "affine_boxes": (np.array(aboxes, np.float64), np.array(abox_mats)),
+            "background": (0.003, 0.006, 0.014)}
+
+
+def overlap_scene(config: BvhBuildConfig | None = None) -> GpuScene:
+    p = overlap_primitives()
+    sc = GpuScene(config or BvhBuildConfig.uniform(LEAF_SIZE))
+    sc.set_background(p["background"])
+    sc.add_spheres(*p["spheres"])
+    sc.add_affine_transforms("box", *p["affine_boxes"])
+    return sc
+
+
+def overlap_camera() -> Camera:
+    """GalleryOverlapScene.create's front camera: SceneSupport.camera(0, 0, 0, 0.125)."""
+    return support_camera((0.0, 0.0, 0.0), 0.0, 0.0, 0.125)

@@ -10,6 +10,7 @@ result files (data only; run in a container where the reference is mounted).
                             (per-frame counters of the uniform and weighted rows of
                             the Hero scales: 1920x1080, 1 spp, depth 12, stack 32;
                             and the weighted-vs-uniform kernel-time reduction)
+  overlap_10k.json       <- the same file's overlap-10k uniform and weighted rows
 """
 import csv
 import json
@@ -36,9 +37,13 @@ def gi_test_topology():
     (OUT / "gi_test_topology.json").write_text(json.dumps(rows, indent=1) + "\n")
 
 
-def _core8_hero_rows():
+def _core8_rows(ids):
     with open(STUDY / "core8-detail.csv") as f:
-        return [r for r in csv.DictReader(f) if r["rowId"] in SCALES and r["methodId"] in ("uniform", "weighted")]
+        return [r for r in csv.DictReader(f) if r["rowId"] in ids and r["methodId"] in ("uniform", "weighted")]
+
+
+def _core8_hero_rows():
+    return _core8_rows(SCALES)
 
 
 def hero_topology():
@@ -53,18 +58,26 @@ def hero_topology():
     (OUT / "hero_topology.json").write_text(json.dumps(cells, indent=0) + "\n")
 
 
+def _counter_row(r, scale):
+    return {"rowId": r["rowId"], "rowOrdinal": int(r["rowOrdinal"]), "scale": scale,
+            "method": r["methodId"], "layoutCount": int(r["layoutCount"]),
+            "topologySha256": r["topologySha256"],
+            "timeReductionPercent": float(r["timeReductionPercent"]),
+            "counters": {k: int(r[c]) for k, c in COUNTERS.items()}}
+
+
 def hero_counters():
-    rows = []
-    for r in _core8_hero_rows():
-        rows.append({"rowId": r["rowId"], "rowOrdinal": int(r["rowOrdinal"]), "scale": SCALES[r["rowId"]],
-                     "method": r["methodId"], "layoutCount": int(r["layoutCount"]),
-                     "topologySha256": r["topologySha256"],
-                     "timeReductionPercent": float(r["timeReductionPercent"]),
-                     "counters": {k: int(r[c]) for k, c in COUNTERS.items()}})
+    rows = [_counter_row(r, SCALES[r["rowId"]]) for r in _core8_hero_rows()]
     (OUT / "hero_counters.json").write_text(json.dumps(rows, indent=1) + "\n")
+
+
+def overlap_rows():
+    rows = [_counter_row(r, 10_000) for r in _core8_rows({"overlap-10k"})]
+    (OUT / "overlap_10k.json").write_text(json.dumps(rows, indent=1) + "\n")
 
 
 if __name__ == "__main__":
     gi_test_topology()
     hero_topology()
     hero_counters()
+    overlap_rows()

@@ -21,6 +21,7 @@ from __future__ import annotations
 import json
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 import mrad
@@ -116,3 +117,32 @@ def test_hero_metrics_frame_counters_near_published(row):
     # every camera pixel starts exactly one segment
     assert ours["rays"] > W.WIDTH * W.HEIGHT
     assert ours["stack_overflows"] == 0 and ours["max_stack_size"] <= W.STACK_CAPACITY
+
+
+OVERLAP_ROWS = json.loads((GOLDEN / "overlap_10k.json").read_text())
+
+
+def test_overlap_population_and_topology_hashes():
+    """GalleryOverlapScene.create (:47-126) over GalleryOverlapSource (:62-151):
+    3,000 spheres and 7,000 affine ribbons; the published uniform and weighted
+    topology SHA-256 of core8-detail.csv."""
+    assert len(OVERLAP_ROWS) == 2
+    prims = W.overlap_primitives()
+    for row in OVERLAP_ROWS:
+        sc = mrad.GpuScene(_config(8, 0.0 if row["method"] == "uniform" else 1.0))
+        sc.set_background(prims["background"])
+        sc.add_spheres(*prims["spheres"])
+        sc.add_affine_transforms("box", *prims["affine_boxes"])
+        c = sc.counts()
+        assert (c["spheres"], c["boxes"], c["affine_boxes"], c["planes"]) == (3000, 0, 7000, 0)
+        assert topology_sha256(sc) == row["topologySha256"], row["method"]
+
+
+def test_overlap_camera_rolls_an_eighth_turn():
+    """SceneSupport.camera(0, 0, 0, 0.125): the benchmark camera's right and up
+    vectors are the unit axes turned by 45 degrees about the view direction."""
+    cam = W.gpu_camera(W.overlap_camera(), W.WIDTH, W.HEIGHT)
+    s = np.sqrt(0.5)
+    np.testing.assert_allclose(cam.right, (s, s, 0.0), atol=1e-12)
+    np.testing.assert_allclose(cam.up, (-s, s, 0.0), atol=1e-12)
+    np.testing.assert_allclose(cam.forward, (0.0, 0.0, 3.0), atol=1e-12)
