@@ -461,7 +461,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["batch", "het", "clustered", "hero"], default="batch")
     ap.add_argument("--queue-capacity", type=int, default=1 << 26)
-    ap.add_argument("--waves-per-cu", type=int, default=32)
+    ap.add_argument("--waves-per-cu", type=int, default=28)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="tune queue capacity, waves per CU and the LDS split on the batch before timing")

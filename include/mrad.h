@@ -220,7 +220,7 @@ typedef struct {
 typedef struct {
 	int device;
 	int queue_capacity;      /* paths in flight per pool (0 = 4M) */
-	int waves_per_cu;        /* trace kernel persistent-grid occupancy (0 = 32) */
+	int waves_per_cu;        /* trace kernel persistent-grid occupancy (0 = 28) */
 	int bvh_stack_size;      /* 8..256 (0 = 32) */
 	int linear_scan;         /* 1 = the reference's GPU_LINEAR scan instead of the BVH */
 	int collect_metrics;     /* counter variant of the trace kernel */

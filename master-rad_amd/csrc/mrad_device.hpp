@@ -556,33 +556,38 @@ __device__ __forceinline__ float entryDistanceFastNz(float lx, float ly, float l
 // of the per-axis checks would have returned; a zero direction component
 // contributes no bounds, or an empty interval when the origin lies outside
 // the slab (the reference's zero-direction branch).
+//
+// The far-slab widening f(t) = t > 0 ? RN(t * 1.00000072) : t is applied
+// once, to the least far distance of the three axes, instead of per axis:
+// f is non-decreasing (RN of a product with a positive constant is
+// monotone, and f(t) > 0 >= f(t') for t > 0 >= t'), so
+// min(cb, f(b.x), f(b.y), f(b.z)) = min(cb, f(min(b.x, b.y, b.z))).
 __device__ __forceinline__ void slabAxisBf(float oc, float dc, float inv, float lo, float hi, bool nz, float &tMin,
-                                           float &tMax) {
+                                           float &bMin) {
 	const float n0 = lo - oc, n1 = hi - oc;
 	const float q0 = n0 * inv, q1 = n1 * inv;
 	float t0 = fmaf(fmaf(-q0, dc, n0), inv, q0);
 	float t1 = fmaf(fmaf(-q1, dc, n1), inv, q1);
-	const float a = fminf(t0, t1), b0 = fmaxf(t0, t1);
-	const float b = b0 > 0.0f ? b0 * 1.00000072f : b0;
+	const float a = fminf(t0, t1), b = fmaxf(t0, t1);
 	if (nz) {
 		tMin = fmaxf(tMin, a);
-		tMax = fminf(tMax, b);
+		bMin = fminf(bMin, b);
 	} else {
 		const bool zero = dc == 0.0f;
 		const bool outside = oc < lo || oc > hi;
 		tMin = fmaxf(tMin, zero ? (outside ? INFINITY : -INFINITY) : a);
-		tMax = fminf(tMax, zero ? INFINITY : b);
+		bMin = fminf(bMin, zero ? INFINITY : b);
 	}
 }
 template <bool kNz>
 __device__ __forceinline__ float entryDistanceFastBf(float lx, float ly, float lz, float hx, float hy, float hz,
                                                      const Ray &r, float beforeT) {
-	float tMin = kEps;
+	float tMin = kEps, bMin = INFINITY;
 	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
-	float tMax = cb;
-	slabAxisBf(r.o.x, r.d.x, r.inv.x, lx, hx, kNz, tMin, tMax);
-	slabAxisBf(r.o.y, r.d.y, r.inv.y, ly, hy, kNz, tMin, tMax);
-	slabAxisBf(r.o.z, r.d.z, r.inv.z, lz, hz, kNz, tMin, tMax);
+	slabAxisBf(r.o.x, r.d.x, r.inv.x, lx, hx, kNz, tMin, bMin);
+	slabAxisBf(r.o.y, r.d.y, r.inv.y, ly, hy, kNz, tMin, bMin);
+	slabAxisBf(r.o.z, r.d.z, r.inv.z, lz, hz, kNz, tMin, bMin);
+	const float tMax = fminf(cb, bMin > 0.0f ? bMin * 1.00000072f : bMin);
 	return (tMax < tMin || !(tMin <= cb)) ? INFINITY : tMin;
 }
 #ifndef MRAD_ENTRY_BRANCH_FREE
@@ -718,6 +723,11 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 #ifndef MRAD_NODE_PREFETCH
 #define MRAD_NODE_PREFETCH 0
 #endif
+// 1 pushes an interior node's children without a branch per child: both
+// slots are written and the stack size advances by the finite ones.
+#ifndef MRAD_PUSH_BRANCH_FREE
+#define MRAD_PUSH_BRANCH_FREE 1
+#endif
 #ifndef MRAD_SHORT_STACK
 #define MRAD_SHORT_STACK 10
 #endif
@@ -733,33 +743,60 @@ constexpr int kShortStack = MRAD_SHORT_STACK;
 #if MRAD_LDS_STACK_TYPED
 typedef __attribute__((address_space(3))) int LdsInt;
 typedef __attribute__((address_space(3))) float LdsFloat;
+typedef __attribute__((address_space(3))) unsigned long long LdsSlot;
 #else
 typedef int LdsInt;
 typedef float LdsFloat;
+typedef unsigned long long LdsSlot;
+#endif
+// 1 keeps a stack entry's node and entry distance in one 64-bit slot, so a
+// pop is one LDS (or spill) read behind one LDS-or-spill branch instead of
+// two of each.
+#ifndef MRAD_STACK_PACKED
+#define MRAD_STACK_PACKED 1
 #endif
 struct LdsStack {
+#if MRAD_STACK_PACKED
+	LdsSlot *slot;    // lane base, stride 64: node | entry bits << 32
+#else
 	LdsInt *node;     // lane base, stride 64
 	LdsFloat *entry;
+#endif
 };
 
 // kShort is the number of top entries held in LDS (the LDS split); kCap the
 // total storage capacity.
 template <int kCap, int kShort = kShortStack>
 struct TravStack {
-	static constexpr int kSpill = kCap > kShort ? kCap - kShort : 1;
+	// kCap + 1 slots: a branch-free push may write one slot past the limit
+	static constexpr int kSpill = kCap + 1 > kShort ? kCap + 1 - kShort : 1;
 	LdsStack lds;
+#if MRAD_STACK_PACKED
+	unsigned long long spill[kSpill];
+	__device__ __forceinline__ void set(int i, int n, float e) {
+		const unsigned long long v = (uint32_t) n | (unsigned long long) __float_as_uint(e) << 32;
+		if (kShort > 0 && i < kShort) lds.slot[i * 64] = v;
+		else spill[i - kShort] = v;
+	}
+	__device__ __forceinline__ void get(int i, int &n, float &e) const {
+		const unsigned long long v = (kShort > 0 && i < kShort) ? lds.slot[i * 64] : spill[i - kShort];
+		n = (int) (uint32_t) v;
+		e = __uint_as_float((uint32_t) (v >> 32));
+	}
+#else
 	int spillNode[kSpill];
 	float spillEntry[kSpill];
 	__device__ __forceinline__ void set(int i, int n, float e) {
 		if (kShort > 0 && i < kShort) { lds.node[i * 64] = n; lds.entry[i * 64] = e; }
 		else { spillNode[i - kShort] = n; spillEntry[i - kShort] = e; }
 	}
-	__device__ __forceinline__ int node(int i) const {
-		return (kShort > 0 && i < kShort) ? lds.node[i * 64] : spillNode[i - kShort];
+	__device__ __forceinline__ void get(int i, int &n, float &e) const {
+		n = (kShort > 0 && i < kShort) ? lds.node[i * 64] : spillNode[i - kShort];
+		e = (kShort > 0 && i < kShort) ? lds.entry[i * 64] : spillEntry[i - kShort];
 	}
-	__device__ __forceinline__ float entry(int i) const {
-		return (kShort > 0 && i < kShort) ? lds.entry[i * 64] : spillEntry[i - kShort];
-	}
+#endif
+	__device__ __forceinline__ int node(int i) const { int n; float e; get(i, n, e); return n; }
+	__device__ __forceinline__ float entry(int i) const { int n; float e; get(i, n, e); return e; }
 };
 
 // traceBvh (GpuKernelSources.java:844-961) in three parts: bvhBegin inserts
@@ -836,8 +873,9 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 	while (active) {
 		if (!pending) {
 			--size;
-			const int node = b.st.node(size);
-			const float entry = b.st.entry(size);
+			int node;
+			float entry;
+			b.st.get(size, node, entry);
 			const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 			if (entry <= ch) {
 				const float4 *rec = S.nodes + 4 * (size_t) node;
@@ -890,8 +928,8 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			carry = false;
 		} else {
 			--size;
-			node = b.st.node(size);
-			const float entry = b.st.entry(size);
+			float entry;
+			b.st.get(size, node, entry);
 			const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 			if (entry > ch) continue;
 		}
@@ -909,6 +947,20 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		int nearN = nd.x, farN = nd.y;
 		float nearE = le, farE = re;
 		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+#if MRAD_PUSH_BRANCH_FREE
+		// size <= cap - 1 here (a pop, or a carry whose push left room), so
+		// the far slot is in storage; a non-finite far child is written
+		// past the top and never read.  Overflow exactly when the carried
+		// child would not fit: the far push fits, so the maximum reaches cap.
+		const int s1 = size + (isfinite(farE) ? 1 : 0);
+		const bool pn = isfinite(nearE);
+		if (pn && s1 >= cap) { c.maxStack(cap); overflow = true; break; }
+		b.st.set(size, farN, farE);
+		size = s1;
+		c.maxStack(s1 + (pn ? 1 : 0));
+		carry = pn;
+		carryNode = nearN;
+#else
 		if (isfinite(farE)) {
 			if (size >= cap) { overflow = true; break; }
 			b.st.set(size, farN, farE);
@@ -921,12 +973,14 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			carry = true;
 			carryNode = nearN;
 		}
+#endif
 	}
 #else
 	while (!overflow && size > 0) {
 		--size;
-		const int node = b.st.node(size);
-		const float entry = b.st.entry(size);
+		int node;
+		float entry;
+		b.st.get(size, node, entry);
 		const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 		if (entry > ch) continue;
 		const float4 *rec = S.nodes + 4 * (size_t) node;
@@ -948,6 +1002,20 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		int nearN = nd.x, farN = nd.y;
 		float nearE = le, farE = re;
 		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+#if MRAD_PUSH_BRANCH_FREE
+		// Both children are written unconditionally, the nearer one above the
+		// farther if that is finite; a non-finite child's slot lies past the
+		// new top (storage holds cap + 1 entries) and is never read.  size <=
+		// cap - 1 after the pop, so only two finite children overflow, after
+		// the far push brought the maximum to cap.
+		const int s1 = size + (isfinite(farE) ? 1 : 0);
+		const int s2 = s1 + (isfinite(nearE) ? 1 : 0);
+		if (s2 > cap) { c.maxStack(cap); overflow = true; break; }
+		b.st.set(size, farN, farE);
+		b.st.set(s1, nearN, nearE);
+		size = s2;
+		c.maxStack(size);
+#else
 		if (isfinite(farE)) {
 			if (size >= cap) { overflow = true; break; }
 			b.st.set(size, farN, farE);
@@ -960,6 +1028,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			size++;
 			c.maxStack(size);
 		}
+#endif
 	}
 #endif
 	b.size = size;

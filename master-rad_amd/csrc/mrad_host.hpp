@@ -15,6 +15,12 @@
 #include "mrad_device.hpp"
 #include "mrad_error.hpp"
 
+// 1 stores the two children of each BVH node side by side in the arena (see
+// nodeSlots); 0 keeps the scene's node order.
+#ifndef MRAD_NODE_PAIRS
+#define MRAD_NODE_PAIRS 1
+#endif
+
 namespace mrad {
 
 #define MRAD_HIP(expr)                                                                                 \
@@ -112,6 +118,52 @@ struct HostArena {
 	}
 
   private:
+	// Arena slot of every node of a scene's BVH, relative to its (even) base;
+	// returns the number of slots.  With MRAD_NODE_PAIRS the two children of
+	// each interior node take slots 2k and 2k + 1, so they share one 128-byte
+	// line: lanes of a wave that part at a node fetch one line, not two.  The
+	// pairs are laid out depth first from each root (a root alone at an even
+	// slot, one pad slot after it).  Storage order is invisible to traversal:
+	// children, bounds and visiting order are the scene's own.  A BVH that is
+	// not a forest of trees over all its nodes (a node reached twice, a child
+	// index out of range, a node no root reaches) keeps the scene's order, and
+	// the validation that follows reports any bad index.
+	static int nodeSlots(const mrad_scene_arrays *a, std::vector<int> &slot) {
+		const int nc = a->node_count;
+		slot.assign((size_t) nc, -1);
+		int next = 0;
+		bool ok = MRAD_NODE_PAIRS != 0;
+		std::vector<int> todo;
+		for (int r = 0; ok && r < a->root_count; r++) {
+			const int ri = a->root_indices[r];
+			if (ri < 0 || ri >= nc) continue;
+			if (slot[ri] >= 0) { ok = false; break; }
+			slot[ri] = next;
+			next += 2;
+			todo.assign(1, ri);
+			while (ok && !todo.empty()) {
+				const int n = todo.back();
+				todo.pop_back();
+				const int32_t *nd = a->node_data + 4 * (size_t) n;
+				if (nd[0] < 0) continue;
+				const int l = nd[0], rr = nd[1];
+				if (l >= nc || rr < 0 || rr >= nc || l == rr || slot[l] >= 0 || slot[rr] >= 0) { ok = false; break; }
+				slot[l] = next;
+				slot[rr] = next + 1;
+				next += 2;
+				todo.push_back(rr);
+				todo.push_back(l);
+			}
+		}
+		for (int n = 0; ok && n < nc; n++)
+			if (slot[n] < 0) ok = false;
+		if (!ok) {
+			for (int n = 0; n < nc; n++) slot[n] = n;
+			return nc;
+		}
+		return next;
+	}
+
 	int appendScene(const mrad_scene_arrays *a, bool useBvh, int stackLimit, SceneRange &R) {
 		const int nc = a->node_count;
 		if (a->sphere_count < 0 || a->box_count < 0 || a->plane_count < 0 || a->affine_sphere_count < 0 ||
@@ -124,10 +176,14 @@ struct HostArena {
 		    (a->root_count && !a->root_indices) || (a->material_count && !a->materials))
 			return set_error(MRAD_ERR_INVALID_ARGUMENT, "scene arrays must be non-null for non-zero counts");
 		if (a->root_count > 4) return set_error(MRAD_ERR_INVALID_ARGUMENT, "at most 4 BVH roots are supported");
+		// node slots: an even base, so that sibling pairs fill 128-byte lines
+		if (nodes.size() / 4 % 2) nodes.resize(nodes.size() + 4, f4(0, 0, 0, 0));
+		std::vector<int> slot;
+		const int slots = nodeSlots(a, slot);
 		const size_t nodeBase = nodes.size() / 4, refBase = refs.size(), sphBase = spheres.size(),
 		             boxBase = boxes.size() / 2, asBase = affS.size() / 3, abBase = affB.size() / 3,
 		             planeBase = planes.size() / 26, matBase = materials.size() / 4;
-		const size_t sizes[] = {nodeBase + nc, refBase + a->primitive_ref_count, sphBase + a->sphere_count,
+		const size_t sizes[] = {nodeBase + slots, refBase + a->primitive_ref_count, sphBase + a->sphere_count,
 		                        boxBase + a->box_count, asBase + a->affine_sphere_count, abBase + a->affine_box_count,
 		                        matBase + a->material_count};
 		for (size_t s : sizes)
@@ -190,10 +246,10 @@ struct HostArena {
 		// primitive's test record in the 48 bytes an interior node uses for its
 		// children's bounds, and the rebased ref in .y, so testing it needs no
 		// fetch beyond the node record itself.
-		nodes.resize(nodes.size() + (size_t) nc * 4);
+		nodes.resize(nodes.size() + (size_t) slots * 4, f4(0, 0, 0, 0));
 		for (int n = 0; n < nc; n++) {
 			const int32_t *nd = a->node_data + 4 * (size_t) n;
-			float4 *r = nodes.data() + 4 * (nodeBase + n);
+			float4 *r = nodes.data() + 4 * (nodeBase + slot[n]);
 			if (nd[0] >= 0) {
 				if (nd[0] >= nc || nd[1] < 0 || nd[1] >= nc)
 					return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH child index out of range at node " + std::to_string(n));
@@ -202,7 +258,8 @@ struct HostArena {
 				r[0] = f4(L[0], L[1], L[2], L[3]);
 				r[1] = f4(L[4], L[5], Rb[0], Rb[1]);
 				r[2] = f4(Rb[2], Rb[3], Rb[4], Rb[5]);
-				r[3] = f4(ibits(nd[0] + (int) nodeBase), ibits(nd[1] + (int) nodeBase), ibits(nd[2]), ibits(nd[3]));
+				r[3] = f4(ibits(slot[nd[0]] + (int) nodeBase), ibits(slot[nd[1]] + (int) nodeBase), ibits(nd[2]),
+				          ibits(nd[3]));
 			} else {
 				if (nd[3] < 0 || nd[2] < 0 || (int64_t) nd[2] + nd[3] > a->primitive_ref_count)
 					return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH leaf refs out of range at node " + std::to_string(n));
@@ -225,7 +282,7 @@ struct HostArena {
 		for (int r = 0; r < a->root_count; r++) {
 			const int ri = a->root_indices[r];
 			const bool ok = ri >= 0 && ri < nc;
-			R.roots[r] = ok ? ri + (int) nodeBase : -1;
+			R.roots[r] = ok ? slot[ri] + (int) nodeBase : -1;
 			for (int k = 0; k < 6; k++) R.rootBounds[r][k] = ok ? a->node_bounds[6 * (size_t) ri + k] : 0.0f;
 		}
 		R.refBase = (int) refBase;

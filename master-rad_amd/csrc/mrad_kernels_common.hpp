@@ -15,14 +15,22 @@ constexpr int kWaves = kBlock / 64;
 // its 4 waves.
 template <int kShort = kShortStack>
 struct StackLdsT {
+#if MRAD_STACK_PACKED
+	unsigned long long slot[kWaves * (kShort > 0 ? kShort : 1) * 64];
+#else
 	int node[kWaves * (kShort > 0 ? kShort : 1) * 64];
 	float entry[kWaves * (kShort > 0 ? kShort : 1) * 64];
+#endif
 };
 using StackLds = StackLdsT<>;
 template <int kShort>
 __device__ __forceinline__ LdsStack laneStack(StackLdsT<kShort> &s) {
 	const int base = (threadIdx.x >> 6) * kShort * 64 + (threadIdx.x & 63);
+#if MRAD_STACK_PACKED
+	return {(LdsSlot *) (s.slot + base)};
+#else
 	return {(LdsInt *) (s.node + base), (LdsFloat *) (s.entry + base)};
+#endif
 }
 
 __device__ __forceinline__ unsigned long long waveSum(unsigned long long v) {

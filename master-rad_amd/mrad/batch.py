@@ -19,7 +19,7 @@ from .tracer import GpuCamera
 @dataclass
 class WavefrontConfig:
     queue_capacity: int = 1 << 22
-    waves_per_cu: int = 32
+    waves_per_cu: int = 28
     bvh_stack_size: int = 32
     linear_scan: bool = False
     collect_metrics: bool = False
