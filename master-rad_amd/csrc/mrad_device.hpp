@@ -473,11 +473,16 @@ __device__ __forceinline__ bool testAffineBox(const float4 *rec, const Ray &ray,
 	return !(worldT >= limit);
 }
 
-// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds, IEEE division.
+// The cull bound of a query whose best distance is t: a box whose entry lies
+// beyond it is skipped (GpuKernelSources.java:747, :878-880).  It changes only
+// when a leaf improves the best hit, so traversal loops keep it in a register.
+__device__ __forceinline__ float cullBound(float t) { return isfinite(t) ? fmaf(t, 1.0000015f, 1.0e-6f) : t; }
+
+// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds, IEEE division;
+// cb = cullBound(best distance).
 __device__ __forceinline__ float entryDistanceIeee(float lx, float ly, float lz, float hx, float hy, float hz, f3 o,
-                                                   f3 d, float beforeT) {
+                                                   f3 d, float cb) {
 	float tMin = kEps;
-	float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
 	float tMax = cb;
 #define MRAD_ENTRY_AXIS(OC, DC, LO, HI)                        \
 	if (DC == 0.0f) {                                          \
@@ -501,9 +506,8 @@ __device__ __forceinline__ float entryDistanceIeee(float lx, float ly, float lz,
 // distance tells +0 from -0 (tMin >= 1e-4 > 0 and tMax > 0 before every axis),
 // so the zero case needs no select.
 __device__ __forceinline__ float entryDistanceFast(float lx, float ly, float lz, float hx, float hy, float hz,
-                                                   const Ray &r, float beforeT) {
+                                                   const Ray &r, float cb) {
 	float tMin = kEps;
-	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
 	float tMax = cb;
 #define MRAD_ENTRY_AXIS(OC, DC, INV, LO, HI)                                       \
 	if (DC == 0.0f) {                                                              \
@@ -528,9 +532,8 @@ __device__ __forceinline__ float entryDistanceFast(float lx, float ly, float lz,
 // The reciprocal form for a ray with no zero direction component: the
 // reference's zero-direction branch of each axis never runs.
 __device__ __forceinline__ float entryDistanceFastNz(float lx, float ly, float lz, float hx, float hy, float hz,
-                                                     const Ray &r, float beforeT) {
+                                                     const Ray &r, float cb) {
 	float tMin = kEps;
-	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
 	float tMax = cb;
 #define MRAD_ENTRY_AXIS(OC, DC, INV, LO, HI)                                       \
 	{                                                                              \
@@ -562,53 +565,63 @@ __device__ __forceinline__ float entryDistanceFastNz(float lx, float ly, float l
 // f is non-decreasing (RN of a product with a positive constant is
 // monotone, and f(t) > 0 >= f(t') for t > 0 >= t'), so
 // min(cb, f(b.x), f(b.y), f(b.z)) = min(cb, f(min(b.x, b.y, b.z))).
-__device__ __forceinline__ void slabAxisBf(float oc, float dc, float inv, float lo, float hi, bool nz, float &tMin,
-                                           float &bMin) {
+// The near (a) and far (b) slab distances of one axis.
+__device__ __forceinline__ void slabAxisBf(float oc, float dc, float inv, float lo, float hi, bool nz, float &a,
+                                           float &b) {
 	const float n0 = lo - oc, n1 = hi - oc;
 	const float q0 = n0 * inv, q1 = n1 * inv;
-	float t0 = fmaf(fmaf(-q0, dc, n0), inv, q0);
-	float t1 = fmaf(fmaf(-q1, dc, n1), inv, q1);
-	const float a = fminf(t0, t1), b = fmaxf(t0, t1);
-	if (nz) {
-		tMin = fmaxf(tMin, a);
-		bMin = fminf(bMin, b);
-	} else {
+	const float t0 = fmaf(fmaf(-q0, dc, n0), inv, q0);
+	const float t1 = fmaf(fmaf(-q1, dc, n1), inv, q1);
+	a = fminf(t0, t1);
+	b = fmaxf(t0, t1);
+	if (!nz) {
 		const bool zero = dc == 0.0f;
 		const bool outside = oc < lo || oc > hi;
-		tMin = fmaxf(tMin, zero ? (outside ? INFINITY : -INFINITY) : a);
-		bMin = fminf(bMin, zero ? INFINITY : b);
+		a = zero ? (outside ? INFINITY : -INFINITY) : a;
+		b = zero ? INFINITY : b;
 	}
 }
 template <bool kNz>
 __device__ __forceinline__ float entryDistanceFastBf(float lx, float ly, float lz, float hx, float hy, float hz,
-                                                     const Ray &r, float beforeT) {
-	float tMin = kEps, bMin = INFINITY;
-	const float cb = isfinite(beforeT) ? fmaf(beforeT, 1.0000015f, 1.0e-6f) : beforeT;
-	slabAxisBf(r.o.x, r.d.x, r.inv.x, lx, hx, kNz, tMin, bMin);
-	slabAxisBf(r.o.y, r.d.y, r.inv.y, ly, hy, kNz, tMin, bMin);
-	slabAxisBf(r.o.z, r.d.z, r.inv.z, lz, hz, kNz, tMin, bMin);
+                                                     const Ray &r, float cb) {
+	float ax, bx, ay, by, az, bz;
+	slabAxisBf(r.o.x, r.d.x, r.inv.x, lx, hx, kNz, ax, bx);
+	slabAxisBf(r.o.y, r.d.y, r.inv.y, ly, hy, kNz, ay, by);
+	slabAxisBf(r.o.z, r.d.z, r.inv.z, lz, hz, kNz, az, bz);
+	// kEps enters through a two-operand max (an inline literal): a
+	// three-operand max would hold it in a scalar register, which this
+	// kernel's scalar budget pays for with spill reloads inside the loop
+	const float tMin = fmaxf(fmaxf(fmaxf(kEps, ax), ay), az);
+	const float bMin = fminf(fminf(bx, by), bz);
 	const float tMax = fminf(cb, bMin > 0.0f ? bMin * 1.00000072f : bMin);
 	return (tMax < tMin || !(tMin <= cb)) ? INFINITY : tMin;
 }
 #ifndef MRAD_ENTRY_BRANCH_FREE
 #define MRAD_ENTRY_BRANCH_FREE 1
 #endif
-// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds.  kSlab picks
-// the quotient form: kSlabAny by the ray's flag, kSlabFast or kSlabIeee when
-// the caller already knows it (a traversal loop specialised per form keeps
-// one form on its hot path).
+// bvhEntryDistance (GpuKernelSources.java:736-770) on 6 bounds with the cull
+// bound cb = cullBound(best distance).  kSlab picks the quotient form:
+// kSlabAny by the ray's flag, kSlabFast or kSlabIeee when the caller already
+// knows it (a traversal loop specialised per form keeps one form on its hot
+// path).
+template <int kSlab = kSlabAny>
+__device__ __forceinline__ float entryDistanceCb(float lx, float ly, float lz, float hx, float hy, float hz,
+                                                 const Ray &r, float cb) {
+#if MRAD_ENTRY_BRANCH_FREE
+	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastBf<true>(lx, ly, lz, hx, hy, hz, r, cb);
+	if constexpr (kSlab == kSlabFast) return entryDistanceFastBf<false>(lx, ly, lz, hx, hy, hz, r, cb);
+#endif
+	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastNz(lx, ly, lz, hx, hy, hz, r, cb);
+	if constexpr (kSlab == kSlabFast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, cb);
+	if constexpr (kSlab == kSlabIeee) return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, cb);
+	if (r.fast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, cb);
+	return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, cb);
+}
+// The same against the best distance itself.
 template <int kSlab = kSlabAny>
 __device__ __forceinline__ float entryDistance(float lx, float ly, float lz, float hx, float hy, float hz, const Ray &r,
                                                float beforeT) {
-#if MRAD_ENTRY_BRANCH_FREE
-	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastBf<true>(lx, ly, lz, hx, hy, hz, r, beforeT);
-	if constexpr (kSlab == kSlabFast) return entryDistanceFastBf<false>(lx, ly, lz, hx, hy, hz, r, beforeT);
-#endif
-	if constexpr (kSlab == kSlabFastNz) return entryDistanceFastNz(lx, ly, lz, hx, hy, hz, r, beforeT);
-	if constexpr (kSlab == kSlabFast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, beforeT);
-	if constexpr (kSlab == kSlabIeee) return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, beforeT);
-	if (r.fast) return entryDistanceFast(lx, ly, lz, hx, hy, hz, r, beforeT);
-	return entryDistanceIeee(lx, ly, lz, hx, hy, hz, r.o, r.d, beforeT);
+	return entryDistanceCb<kSlab>(lx, ly, lz, hx, hy, hz, r, cullBound(beforeT));
 }
 
 // ------------------------------------------------------------ closest hit
@@ -860,6 +873,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 	const int cap = R.stackLimit;
 	int size = b.size;
 	bool overflow = b.overflow;
+	float ch = cullBound(best.t);   // changes only when a leaf improves best.t
 #if MRAD_LEAF_BATCH > 0
 	// The same per-lane sequence (pop, then expand an interior node or test a
 	// leaf's refs before the next pop), but a lane that pops a leaf holds it
@@ -876,9 +890,8 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			int node;
 			float entry;
 			b.st.get(size, node, entry);
-			const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 			if (entry <= ch) {
-				const float4 *rec = S.nodes + 4 * (size_t) node;
+				const float4 *rec = S.nodes + 4 * (size_t) (uint32_t) node;
 				const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
 				if (nd.x < 0) {
 					leaf = nd;
@@ -887,8 +900,8 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 				} else {
 					c.add(10, 1);
 					const float4 a = rec[0], bb = rec[1], cc = rec[2];
-					const float le = entryDistance<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
-					const float re = entryDistance<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
+					const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
+					const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
 					c.add(1, 2);
 					int nearN = nd.x, farN = nd.y;
 					float nearE = le, farE = re;
@@ -908,6 +921,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		if (waiting == __ballot(1) || __popcll(waiting) >= MRAD_LEAF_BATCH) {
 			if (pending) {
 				testLeaf<kMetrics, kSlab>(S, R, S.nodes + 4 * (size_t) leafNode, leaf, ray, best, c);
+				ch = cullBound(best.t);
 				pending = false;
 			}
 		}
@@ -930,19 +944,19 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			--size;
 			float entry;
 			b.st.get(size, node, entry);
-			const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 			if (entry > ch) continue;
 		}
-		const float4 *rec = S.nodes + 4 * (size_t) node;
+		const float4 *rec = S.nodes + 4 * (size_t) (uint32_t) node;
 		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
 		if (nd.x < 0) {
 			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
+			ch = cullBound(best.t);
 			continue;
 		}
 		c.add(10, 1);
 		const float4 a = rec[0], bb = rec[1], cc = rec[2];
-		const float le = entryDistance<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
-		const float re = entryDistance<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
+		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
+		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
 		c.add(1, 2);
 		int nearN = nd.x, farN = nd.y;
 		float nearE = le, farE = re;
@@ -981,23 +995,23 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		int node;
 		float entry;
 		b.st.get(size, node, entry);
-		const float ch = isfinite(best.t) ? fmaf(best.t, 1.0000015f, 1.0e-6f) : best.t;
 		if (entry > ch) continue;
-		const float4 *rec = S.nodes + 4 * (size_t) node;
+		const float4 *rec = S.nodes + 4 * (size_t) (uint32_t) node;
 #if MRAD_NODE_PREFETCH
 		const float4 a = rec[0], bb = rec[1], cc = rec[2];
 #endif
 		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
 		if (nd.x < 0) {
 			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
+			ch = cullBound(best.t);
 			continue;
 		}
 		c.add(10, 1);
 #if !MRAD_NODE_PREFETCH
 		const float4 a = rec[0], bb = rec[1], cc = rec[2];
 #endif
-		const float le = entryDistance<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, best.t);
-		const float re = entryDistance<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, best.t);
+		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
+		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
 		c.add(1, 2);
 		int nearN = nd.x, farN = nd.y;
 		float nearE = le, farE = re;
