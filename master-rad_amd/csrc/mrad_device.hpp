@@ -864,6 +864,18 @@ __device__ __forceinline__ void bvhEnd(const DevScene &S, const SceneRange &R, c
 	}
 }
 
+// The children of an interior node in visiting order, the nearer first and
+// the left one on a tie (GpuKernelSources.java:915-922), as selects rather
+// than a swap under a branch.
+__device__ __forceinline__ void orderChildren(int4 nd, float le, float re, int &nearN, int &farN, float &nearE,
+                                              float &farE) {
+	const bool swap = re < le;
+	nearN = swap ? nd.y : nd.x;
+	farN = swap ? nd.x : nd.y;
+	nearE = swap ? re : le;
+	farE = swap ? le : re;
+}
+
 template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny>
 __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
                                 Counters<kMetrics> &c, LdsStack lds) {
@@ -903,9 +915,9 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 					const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
 					const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
 					c.add(1, 2);
-					int nearN = nd.x, farN = nd.y;
-					float nearE = le, farE = re;
-					if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+					int nearN, farN;
+					float nearE, farE;
+					orderChildren(nd, le, re, nearN, farN, nearE, farE);
 					if (isfinite(farE)) {
 						if (size >= cap) overflow = true;
 						else { b.st.set(size, farN, farE); size++; c.maxStack(size); }
@@ -958,9 +970,9 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
 		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
 		c.add(1, 2);
-		int nearN = nd.x, farN = nd.y;
-		float nearE = le, farE = re;
-		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+		int nearN, farN;
+		float nearE, farE;
+		orderChildren(nd, le, re, nearN, farN, nearE, farE);
 #if MRAD_PUSH_BRANCH_FREE
 		// size <= cap - 1 here (a pop, or a carry whose push left room), so
 		// the far slot is in storage; a non-finite far child is written
@@ -1013,9 +1025,9 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
 		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
 		c.add(1, 2);
-		int nearN = nd.x, farN = nd.y;
-		float nearE = le, farE = re;
-		if (re < le) { nearN = nd.y; farN = nd.x; nearE = re; farE = le; }
+		int nearN, farN;
+		float nearE, farE;
+		orderChildren(nd, le, re, nearN, farN, nearE, farE);
 #if MRAD_PUSH_BRANCH_FREE
 		// Both children are written unconditionally, the nearer one above the
 		// farther if that is finite; a non-finite child's slot lies past the
