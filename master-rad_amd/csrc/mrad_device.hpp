@@ -727,10 +727,13 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 #ifndef MRAD_SPLIT_SLAB
 #define MRAD_SPLIT_SLAB 2
 #endif
-// 1 carries the nearer child to the next iteration in registers (megakernel
-// 8.8 %, adjoint backward 3 % faster on the same GPU).
+// 1 carries the nearer child to the next iteration in registers instead of
+// pushing and popping it.  With the packed stack, the branch-free push and
+// the select-ordered children, pushing both is faster in every kernel
+// (megakernel 1.6 %, adjoint backward 2.3 %, wavefront trace 3.1 %): the
+// carried and the popping lanes of a wave take different paths.
 #ifndef MRAD_CARRY_NEAR
-#define MRAD_CARRY_NEAR 1
+#define MRAD_CARRY_NEAR 0
 #endif
 // 1 fetches a popped node's whole 64-byte record before its kind is known.
 #ifndef MRAD_NODE_PREFETCH

@@ -18,16 +18,6 @@
 // both forms produce bit-identical images.
 #include <hip/hip_runtime.h>
 
-// The trace kernel is compiled to the 72-VGPR budget of 7 waves per SIMD
-// with the traversal stack's entries packed into 64-bit slots and the
-// branch-free push (1.8 % faster than the 64-VGPR budget of 8 waves without
-// them, where both spill live values inside the loop).  Carrying the nearer
-// child in registers, as the other kernels do, makes it 3.5 % slower, so its
-// traversal pushes both children.
-#ifndef MRAD_CARRY_NEAR
-#define MRAD_CARRY_NEAR 0
-#endif
-
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -168,7 +158,9 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 }
 
 // Occupancy the trace kernels are compiled for (waves per SIMD; the register
-// budget is 512 / this).
+// budget is 512 / this).  7 waves (72 VGPRs) with the packed stack slots and
+// the branch-free push is 1.8 % faster than 8 waves (64 VGPRs), where both
+// spill live values inside the traversal loop.
 #ifndef MRAD_TRACE_WAVES_PER_EU
 #define MRAD_TRACE_WAVES_PER_EU 7
 #endif
