@@ -62,6 +62,15 @@ class BvhBuildConfig:
                               sphere_weight=w["sphere"], box_weight=w["box"],
                               affine_sphere_weight=w["affineSphere"], affine_box_weight=w["affineBox"], lam=lam)
 
+    def summary(self) -> str:
+        """BvhBuildConfig.summary (BvhBuildConfig.java:71-76), the recorder's
+        accelerationConfig."""
+        from .recorder import java_fixed
+        w = (self.traversal_weight, self.sphere_weight, self.box_weight, self.affine_sphere_weight,
+             self.affine_box_weight, self.lam)
+        return ("mode={},leafSize={},interiorTraversal={},sphere={},box={},affineSphere={},affineBox={},lambda={}"
+                .format(self.mode, self.leaf_size, *(java_fixed(x, 6) for x in w)))
+
     def native(self) -> N.BvhConfig:
         modes = {"UNIFORM_SAH": N.MRAD_BVH_UNIFORM_SAH, "WEIGHTED_SAH": N.MRAD_BVH_WEIGHTED_SAH}
         if self.mode not in modes:

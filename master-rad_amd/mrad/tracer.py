@@ -227,7 +227,8 @@ class GpuRayTracer:
 
 class HeadlessGpuRenderer:
     """HeadlessGpuRenderer.render (HeadlessGpuRenderer.java:58-148): per-frame
-    seeds, one renderSample per frame, FP64 accumulation of frame * spp."""
+    seeds, one renderSample per frame, FP64 accumulation of frame * spp, and
+    with a recorder the run metadata and one benchmark row per measured frame."""
 
     def __init__(self, scene: GpuScene, camera: GpuCamera, width: int, height: int, max_depth: int, seed: int,
                  samples_per_frame: int = 1, bvh_traversal: bool = True, rr_start: int = -1, **tracer_kw):
@@ -235,15 +236,67 @@ class HeadlessGpuRenderer:
         self.scene, self.camera = scene, camera
         self.width, self.height, self.max_depth, self.seed = width, height, max_depth, seed
         self.samples_per_frame = samples_per_frame
+        self.bvh_traversal = bvh_traversal
+        self.collect_metrics = bool(tracer_kw.get("collect_metrics", False))
         self.tracer = GpuRayTracer(width, height, samples_per_frame, bvh_traversal, **tracer_kw)
         if rr_start >= 0:
             self.tracer.set_russian_roulette(rr_start)
 
-    def render(self, frames: int, warmup_frames: int = 0) -> np.ndarray:
+    def _run_metadata(self, frame_floats: int):
+        """The RunMetadata of HeadlessGpuRenderer.java:65-88."""
+        from .recorder import RunMetadata, scene_metrics
+        m = scene_metrics(self.scene)
+        st = self.scene.bvh_stats() if self.bvh_traversal else None
+        bvh_bytes = 0 if st is None else int(st.bytes)
+        return RunMetadata(
+            m.scene_bytes, bvh_bytes, frame_floats * 8 + m.scene_bytes + bvh_bytes,
+            frame_floats * 4 + m.scene_bytes + bvh_bytes + 14 * 8,
+            m.primitive_count, m.sphere_count, m.box_count, m.plane_count, m.affine_sphere_count, m.affine_box_count,
+            m.material_count, 0 if st is None else st.node_count, 0 if st is None else st.primitive_ref_count,
+            0 if st is None else st.leaf_count, 0 if st is None else st.max_depth,
+            None if st is None else st.leaf_size, None if st is None else st.min_leaf_occupancy,
+            None if st is None else st.max_leaf_occupancy, None if st is None else float(st.mean_leaf_occupancy),
+            None if st is None else float(st.generalized_sah_cost), None if st is None else float(st.uniform_sah_cost),
+            None if st is None else float(st.weighted_sah_cost),
+            self.scene.config.summary() if self.bvh_traversal else "mode=LINEAR")
+
+    def render(self, frames: int, warmup_frames: int = 0, progress_every: int = 0, recorder=None) -> np.ndarray:
+        import time
+        from .recorder import FrameMetrics, root_aabb_tests, traversal_work_of
         frame = np.zeros(self.width * self.height * 3, np.float32)
         sums = np.zeros(self.width * self.height * 3, np.float64)
+        if recorder is not None:
+            recorder.set_run_metadata(self._run_metadata(frame.size))
+            recorder.set_compiled_code_object_sha256(self.compiled_code_object_sha256())
         for fi in range(max(0, warmup_frames) + max(0, frames)):
+            t0 = time.perf_counter_ns()
             self.tracer.render_sample(frame, self.scene, self.camera, self.max_depth, fi, frame_seed(self.seed, fi))
-            if fi >= warmup_frames:
+            measured = fi >= warmup_frames
+            if measured:
                 sums += frame.astype(np.float64) * self.samples_per_frame
+            wall = time.perf_counter_ns() - t0
+            st = self.tracer.last_frame_stats()
+            if measured and warmup_frames > 0 and st["upload_nanos"] != 0:
+                raise N.IllegalState(N.MRAD_ERR_ILLEGAL_STATE,
+                                     "Measured steady-state GPU frame unexpectedly uploaded scene data")
+            if measured and recorder is not None:
+                m = self.collect_metrics
+                pick = (lambda k: st[k] if m else None)
+                recorder.record(FrameMetrics(
+                    self.samples_per_frame, wall, st["kernel_nanos"], st["maximum_physical_kernel_nanos"],
+                    st["copy_nanos"], st["upload_nanos"], self.width, self.height,
+                    self.width * self.height * self.samples_per_frame if m else None, pick("rays"),
+                    pick("primitive_tests"), pick("aabb_tests"), pick("sphere_tests"), pick("box_tests"),
+                    pick("plane_tests"), pick("affine_sphere_tests"), pick("affine_box_tests"),
+                    root_aabb_tests(st) if m else None, traversal_work_of(st) if m else None,
+                    pick("stack_overflows"), pick("max_stack_size"), pick("internal_node_visits"),
+                    pick("leaf_node_visits"), pick("homogeneous_leaf_node_visits"), pick("mixed_leaf_node_visits"),
+                    st["total_nanos"]))
+            done = fi - warmup_frames + 1
+            if measured and progress_every > 0 and done % progress_every == 0:
+                print(f"  progress {done}/{frames}", flush=True)
         return sums / max(1, frames * self.samples_per_frame)
+
+    @staticmethod
+    def compiled_code_object_sha256() -> str:
+        return GpuRayTracer.compiled_code_object_sha256()

@@ -11,6 +11,11 @@ result files (data only; run in a container where the reference is mounted).
                             the Hero scales: 1920x1080, 1 spp, depth 12, stack 32;
                             and the weighted-vs-uniform kernel-time reduction)
   overlap_10k.json       <- the same file's overlap-10k uniform and weighted rows
+  bvh_matrix.json        <- bvh-node-count-matrix.csv, bvh-depth-matrix.csv,
+                            bvh-bytes-matrix.csv (uniform and weighted columns of
+                            the Hero and overlap rows: nodes, depth, packed bytes)
+  modeled_work.json      <- core8-detail.csv, every single-layout row (all eight
+                            methods): the six test counters and medianModeledWork
 """
 import csv
 import json
@@ -76,8 +81,36 @@ def overlap_rows():
     (OUT / "overlap_10k.json").write_text(json.dumps(rows, indent=1) + "\n")
 
 
+def bvh_matrix():
+    tables = {}
+    for key, name in (("nodeCount", "bvh-node-count-matrix.csv"), ("maxDepth", "bvh-depth-matrix.csv"),
+                      ("bytes", "bvh-bytes-matrix.csv")):
+        with open(STUDY / name) as f:
+            tables[key] = {r["rowId"]: r for r in csv.DictReader(f)}
+    rows = []
+    for row_id in list(SCALES) + ["overlap-10k"]:
+        for method in ("uniform", "weighted"):
+            rows.append({"rowId": row_id, "method": method,
+                         **{k: int(t[row_id][method]) for k, t in tables.items()}})
+    (OUT / "bvh_matrix.json").write_text(json.dumps(rows, indent=0) + "\n")
+
+
+def modeled_work():
+    tests = ("medianAabbTests", "medianSphereTests", "medianBoxTests", "medianPlaneTests",
+             "medianAffineSphereTests", "medianAffineBoxTests")
+    rows = []
+    with open(STUDY / "core8-detail.csv") as f:
+        for r in csv.DictReader(f):
+            if r["layoutCount"] == "1":
+                rows.append({"rowId": r["rowId"], "method": r["methodId"],
+                             **{k: int(r[k]) for k in tests}, "medianModeledWork": r["medianModeledWork"]})
+    (OUT / "modeled_work.json").write_text(json.dumps(rows, indent=0) + "\n")
+
+
 if __name__ == "__main__":
     gi_test_topology()
     hero_topology()
     hero_counters()
     overlap_rows()
+    bvh_matrix()
+    modeled_work()

@@ -1,0 +1,59 @@
+"""GPU: HeadlessGpuRenderer with a BenchmarkRecorder, as
+HeadlessGpuRenderer.render drives it (playground/HeadlessGpuRenderer.java:58-148):
+run metadata from the scene and its BVH, the code-object hash, and one row per
+measured frame whose counters are the oracle's for that frame's seed."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import mrad
+from mrad import recorder as R
+from mrad import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_headless_renderer_writes_reference_rows(require_gpu, tmp_path):
+    sc = W.hero_scene(96, mrad.BvhBuildConfig.weighted(8, 1.0))
+    cam = W.hero_gpu_camera()
+    width, height, seed = 320, 180, 7
+    out = tmp_path / "hero-96.csv"
+    print("headless hero-96 320x180, 1 warm-up + 2 recorded frames", flush=True)
+    rec = R.BenchmarkRecorder(out, flush_every=1, gpu_info="AMD Instinct MI355X", scene_name="hero-96", echo=False)
+    r = mrad.HeadlessGpuRenderer(sc, cam, width, height, W.MAXIMUM_PATH_DEPTH, seed, collect_metrics=True,
+                                 bvh_stack_size=W.STACK_CAPACITY)
+    img = r.render(2, warmup_frames=1, recorder=rec)
+    last = r.tracer.last_frame_stats()
+    rec.close()
+    assert np.isfinite(img).all()
+
+    lines = out.read_text().splitlines()
+    st = sc.bvh_stats()
+    head = lines[0]
+    for item in ("metricSchemaVersion=5", f"bvhBytes={st.bytes}", f"accelerationNodeCount={st.node_count}",
+                 f"leafCount={st.leaf_count}", f"maxTreeDepth={st.max_depth}", "primitiveCount=105",
+                 f"accelerationConfig={sc.config.summary()}",
+                 f"compiledPtxSha256={mrad.GpuRayTracer.compiled_code_object_sha256()}"):
+        assert item in head, item
+    assert lines[1] == ",".join(R.COLUMNS)
+    rows = [dict(zip(R.COLUMNS, line.split(","))) for line in lines[2:]]
+    assert len(rows) == 2 and all(len(line.split(",")) == 34 for line in lines[2:])
+
+    # the last row is frame 2: its counters are the oracle's for that frame
+    _, ref = O.render_pixels(O.Scene(sc.wire(), stack_size=W.STACK_CAPACITY), cam.packed(), width, height,
+                             W.MAXIMUM_PATH_DEPTH, 2, mrad.frame_seed(seed, 2), 1)
+    ref = dict(zip(mrad.STAT_NAMES, map(int, ref)))
+    assert {k: last[k] for k in mrad.STAT_NAMES} == ref
+    row = rows[-1]
+    for col, key in (("pathSegments", "rays"), ("primitiveTests", "primitive_tests"), ("aabbTests", "aabb_tests"),
+                     ("internalNodeVisits", "internal_node_visits"), ("leafNodeVisits", "leaf_node_visits"),
+                     ("sphereTests", "sphere_tests"), ("boxTests", "box_tests"),
+                     ("affineBoxTests", "affine_box_tests"), ("maxStackSize", "max_stack_size")):
+        assert int(row[col]) == ref[key], col
+    assert int(row["cameraSamples"]) == width * height
+    assert int(row["secondaryPathSegments"]) == ref["rays"] - width * height
+    assert int(row["rootAabbTests"]) == ref["aabb_tests"] - 2 * ref["internal_node_visits"]
+    assert row["traversalWork"] == R.java_fixed(R.traversal_work_of(ref), 4)
+    assert row["uploadMs"] == "0.0000" and float(row["kernelMs"]) > 0.0
