@@ -374,7 +374,10 @@ def bench_hero(args) -> dict:
     uniform and weighted (lambda 1) SAH at leaf 8.  Reported per scale: kernel
     ms per frame (median), segments per second, and the weighted-vs-uniform
     kernel-time reduction beside the reference's published one (RTX 3080,
-    core8-detail.csv).  The line's value is the 1M-primitive weighted frame."""
+    core8-detail.csv).  Each frame is timed with the megakernel's leaf
+    batching off (the reduction columns, the reference's traversal order of
+    work) and on (16 lanes; same images and counters); the line's value is the
+    1M-primitive weighted frame in the faster of the two."""
     from mrad import workloads as W
     pub = {(r["rowId"], r["method"]): r
            for r in json.loads((ROOT / "tests" / "golden" / "hero_counters.json").read_text())}
@@ -389,28 +392,36 @@ def bench_hero(args) -> dict:
             cfg = mrad.BvhBuildConfig.uniform(W.LEAF_SIZE) if lam == 0.0 else mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, lam)
             sc = W.hero_scene(scale, cfg)
             base = W.timing_seed(key, 0)
-            with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
-                for i in range(max(1, args.warmup)):
-                    t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
-                kern = []
-                t0 = time.perf_counter()
-                for i in range(frames):
-                    t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
-                    kern.append(t.last_frame_stats()["kernel_nanos"])
-                wall = (time.perf_counter() - t0) / frames
+            timed = {}
+            for lanes in (0, 16):
+                with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
+                    t.set_leaf_batch(lanes)
+                    for i in range(max(1, args.warmup)):
+                        t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
+                    kern = []
+                    t0 = time.perf_counter()
+                    for i in range(frames):
+                        t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
+                        kern.append(t.last_frame_stats()["kernel_nanos"])
+                    timed[lanes] = (float(np.median(kern)) / 1e6, (time.perf_counter() - t0) / frames)
             with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, collect_metrics=True,
                                    bvh_stack_size=W.STACK_CAPACITY) as tm:
                 tm.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, 0, W.metrics_seed(key))
                 st = tm.last_frame_stats()
-            k_ms = float(np.median(kern)) / 1e6
-            res[method] = {"kernel_ms": k_ms, "wall_ms": wall * 1e3, "segments": st["rays"],
+            best = min(timed, key=lambda k: timed[k][0])
+            k_ms, wall = timed[best]
+            res[method] = {"kernel_ms": timed[0][0], "kernel_ms_leaf_batch": timed[16][0], "leaf_batch": best,
+                           "best_kernel_ms": k_ms, "wall_ms": wall * 1e3, "segments": st["rays"],
                            "gsegments_per_s": st["rays"] / (k_ms * 1e6),
                            "roofline_gbs": (algorithmic_bytes(st) + 12.0 * W.WIDTH * W.HEIGHT) / (k_ms * 1e6),
                            "counters": st}
-            log(f"{rid} {method}: {k_ms:.3f} ms kernel, {wall * 1e3:.3f} ms wall, {st['rays']} segments")
+            log(f"{rid} {method}: {timed[0][0]:.3f} ms kernel ({timed[16][0]:.3f} ms with leaf batching), "
+                f"{wall * 1e3:.3f} ms wall, {st['rays']} segments")
         red = 100.0 * (1.0 - res["weighted"]["kernel_ms"] / res["uniform"]["kernel_ms"])
         rows.append({"row": rid, "primitives": scale + 9,
                      "uniform_kernel_ms": res["uniform"]["kernel_ms"], "weighted_kernel_ms": res["weighted"]["kernel_ms"],
+                     "uniform_kernel_ms_leaf_batch": res["uniform"]["kernel_ms_leaf_batch"],
+                     "weighted_kernel_ms_leaf_batch": res["weighted"]["kernel_ms_leaf_batch"],
                      "weighted_gsegments_per_s": res["weighted"]["gsegments_per_s"],
                      "weighted_time_reduction_pct": red,
                      "published_rtx3080_time_reduction_pct": pub[(rid, "weighted")]["timeReductionPercent"]})
@@ -420,20 +431,24 @@ def bench_hero(args) -> dict:
     ovl = {(r["rowId"], r["method"]): r for r in json.loads((ROOT / "tests" / "golden" / "overlap_10k.json").read_text())}
     ocam = W.gpu_camera(W.overlap_camera(), W.WIDTH, W.HEIGHT)
     okey = W.workload_key("overlap-10k")
-    oms = {}
+    oms, olb = {}, {}
     for method, lam in (("uniform", 0.0), ("weighted", 1.0)):
         cfg = mrad.BvhBuildConfig.uniform(W.LEAF_SIZE) if lam == 0.0 else mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, lam)
         sc = W.overlap_scene(cfg)
-        with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
-            t.render_sample(None, sc, ocam, W.MAXIMUM_PATH_DEPTH, 0, W.timing_seed(okey, 0))
-            kern = []
-            for i in range(max(3, frames // 4)):
-                t.render_sample(None, sc, ocam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(W.timing_seed(okey, 0), i))
-                kern.append(t.last_frame_stats()["kernel_nanos"])
-        oms[method] = float(np.median(kern)) / 1e6
-        log(f"overlap-10k {method}: {oms[method]:.2f} ms kernel")
+        for lanes in (0, 16):
+            with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
+                t.set_leaf_batch(lanes)
+                t.render_sample(None, sc, ocam, W.MAXIMUM_PATH_DEPTH, 0, W.timing_seed(okey, 0))
+                kern = []
+                for i in range(max(3, frames // 4)):
+                    t.render_sample(None, sc, ocam, W.MAXIMUM_PATH_DEPTH, i,
+                                    mrad.frame_seed(W.timing_seed(okey, 0), i))
+                    kern.append(t.last_frame_stats()["kernel_nanos"])
+            (olb if lanes else oms)[method] = float(np.median(kern)) / 1e6
+        log(f"overlap-10k {method}: {oms[method]:.2f} ms kernel ({olb[method]:.2f} ms with leaf batching)")
     rows.append({"row": "overlap-10k", "primitives": W.OVERLAP_OBJECTS, "uniform_kernel_ms": oms["uniform"],
-                 "weighted_kernel_ms": oms["weighted"],
+                 "weighted_kernel_ms": oms["weighted"], "uniform_kernel_ms_leaf_batch": olb["uniform"],
+                 "weighted_kernel_ms_leaf_batch": olb["weighted"],
                  "weighted_time_reduction_pct": 100.0 * (1.0 - oms["weighted"] / oms["uniform"]),
                  "published_rtx3080_time_reduction_pct": ovl[("overlap-10k", "weighted")]["timeReductionPercent"]})
     paths = W.WIDTH * W.HEIGHT
@@ -444,12 +459,13 @@ def bench_hero(args) -> dict:
         "data": "synthetic (the reference's deterministic Layered Hero generator)",
         "config": {"workload": "hero-1m weighted megakernel (LayeredHeroFamily 1M + 9 environment primitives)",
                    "width": W.WIDTH, "height": W.HEIGHT, "spp": 1, "max_depth": W.MAXIMUM_PATH_DEPTH,
-                   "bvh": "weighted SAH lambda 1 leaf 8", "stack": W.STACK_CAPACITY},
+                   "bvh": "weighted SAH lambda 1 leaf 8", "stack": W.STACK_CAPACITY,
+                   "leaf_batch": line["leaf_batch"]},
         "mrays_per_s": line["segments"] / (line["wall_ms"] / 1e3) / 1e6,
         "scales": rows,
         "roofline": {"bound": "hbm", "achieved": line["roofline_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": line["roofline_gbs"] / HBM_PEAK_GBS, "traffic": load_traffic("hero-1m"),
-                     "kernel": "renderKernel", "kernel_ms_per_step": line["kernel_ms"]},
+                     "kernel": "renderKernel", "kernel_ms_per_step": line["best_kernel_ms"]},
         "counters_per_step": {k: line["counters"][k] for k in mrad.STAT_NAMES},
     }, [configs.Workload("hero-1m", line["scene"], cam, W.WIDTH, W.HEIGHT, 1, W.MAXIMUM_PATH_DEPTH)]
 

@@ -158,6 +158,13 @@ int mrad_upload_scene(mrad_tracer *tracer, const mrad_scene_arrays *scene, uint6
  * 0.05, 0.95); -1 (the default) renders exactly the reference integrator.
  * No counterpart in the reference: its integrator has no roulette. */
 int mrad_set_russian_roulette(mrad_tracer *tracer, int start_depth);
+/* Leaf batching of the megakernel's traversal: 0 (the default) tests a leaf
+ * as soon as a lane pops it; 16 holds popped leaves until 16 lanes of the
+ * wave (or every lane still traversing) hold one and tests them together.
+ * Hits, images and counters are the same either way; leaf-heavy scenes (the
+ * Layered Hero frames) render faster with 16, the seeded fields without.
+ * No counterpart in the reference.  Other values: invalid argument. */
+int mrad_set_leaf_batch(mrad_tracer *tracer, int lanes);
 /* GpuRayTracer.renderSample(rgb, scene, camera, maxDepth, frameIndex,
  * frameSeed) (GpuRayTracer.java:317-432).  camera = position, forward,
  * right, up (GpuCamera.java:8-22).  rgb_out (W*H*3 floats) may be NULL to

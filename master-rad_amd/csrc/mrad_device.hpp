@@ -715,13 +715,10 @@ __device__ __forceinline__ void testLeaf(const DevScene &S, const SceneRange &R,
 // accesses hit 64 distinct banks whatever each lane's depth; deeper entries
 // spill to a thread-private array.  Capacity and overflow semantics are the
 // reference's BVH_STACK_SIZE (GpuKernelSources.java:858-859, :871, :924, :933).
-// Leaf batching of the traversal loop: 0 tests a leaf's refs as soon as a
-// lane pops it; K > 0 holds popped leaves until K lanes of the wave (or all
-// that still traverse) hold one.  Both visit nodes and test refs in the same
-// per-lane order, so hits and counters are identical.
-#ifndef MRAD_LEAF_BATCH
-#define MRAD_LEAF_BATCH 0
-#endif
+// Leaf batching of the traversal loop (traceBvh's kBatch): 0 tests a leaf's
+// refs as soon as a lane pops it; K > 0 holds popped leaves until K lanes of
+// the wave (or all that still traverse) hold one.  Both visit nodes and test
+// refs in the same per-lane order, so hits and counters are identical.
 // 1 specialises the traversal loop per slab-quotient form (see traceClosest);
 // 2 adds a loop for reciprocal-form rays with no zero direction component.
 #ifndef MRAD_SPLIT_SLAB
@@ -879,7 +876,7 @@ __device__ __forceinline__ void orderChildren(int4 nd, float le, float re, int &
 	farE = swap ? le : re;
 }
 
-template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny>
+template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny, int kBatch = 0>
 __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
                                 Counters<kMetrics> &c, LdsStack lds) {
 	BvhState<kCap, kShort> b;
@@ -889,12 +886,12 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 	int size = b.size;
 	bool overflow = b.overflow;
 	float ch = cullBound(best.t);   // changes only when a leaf improves best.t
-#if MRAD_LEAF_BATCH > 0
+	if constexpr (kBatch > 0) {
 	// The same per-lane sequence (pop, then expand an interior node or test a
 	// leaf's refs before the next pop), but a lane that pops a leaf holds it
-	// until MRAD_LEAF_BATCH lanes of the wave hold one (or every lane still
-	// traversing does), and those leaves are tested together: the leaf branch
-	// then runs with many active lanes instead of one or two.
+	// until kBatch lanes of the wave hold one (or every lane still traversing
+	// does), and those leaves are tested together: the leaf branch then runs
+	// with many active lanes instead of one or two.
 	bool active = !overflow && size > 0;
 	bool pending = false;
 	int4 leaf = make_int4(0, 0, 0, 0);
@@ -933,7 +930,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 			}
 		}
 		const unsigned long long waiting = __ballot(pending);
-		if (waiting == __ballot(1) || __popcll(waiting) >= MRAD_LEAF_BATCH) {
+		if (waiting == __ballot(1) || __popcll(waiting) >= kBatch) {
 			if (pending) {
 				testLeaf<kMetrics, kSlab>(S, R, S.nodes + 4 * (size_t) leafNode, leaf, ray, best, c);
 				ch = cullBound(best.t);
@@ -942,7 +939,8 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		}
 		active = !overflow && (pending || size > 0);
 	}
-#elif MRAD_CARRY_NEAR
+	} else {
+#if MRAD_CARRY_NEAR
 	// The nearer child is carried to the next iteration in registers instead
 	// of being pushed and popped at once.  Same order and counters: the
 	// overflow check and the stack-size maximum are taken as if it were
@@ -1060,6 +1058,7 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 #endif
 	}
 #endif
+	}
 	b.size = size;
 	b.overflow = overflow;
 	bvhEnd<kMetrics>(S, R, ray, best, c, b);
@@ -1181,7 +1180,7 @@ __device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, float t, i
 }
 
 // traceClosest (GpuKernelSources.java:963-1020).
-template <bool kMetrics, int kCap, int kShort = kShortStack>
+template <bool kMetrics, int kCap, int kShort = kShortStack, int kBatch = 0>
 __device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
                                              Counters<kMetrics> &c, LdsStack lds) {
 	best.t = 1e30f;
@@ -1195,14 +1194,14 @@ __device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange
 	if (R.useBvh != 0) {
 #if MRAD_SPLIT_SLAB > 1
 		if (ray.fast && d.x != 0.0f && d.y != 0.0f && d.z != 0.0f)
-			traceBvh<kMetrics, kCap, kShort, kSlabFastNz>(S, R, ray, best, c, lds);
+			traceBvh<kMetrics, kCap, kShort, kSlabFastNz, kBatch>(S, R, ray, best, c, lds);
 		else
 #endif
-		if (ray.fast) traceBvh<kMetrics, kCap, kShort, kSlabFast>(S, R, ray, best, c, lds);
-		else traceBvh<kMetrics, kCap, kShort, kSlabIeee>(S, R, ray, best, c, lds);
+		if (ray.fast) traceBvh<kMetrics, kCap, kShort, kSlabFast, kBatch>(S, R, ray, best, c, lds);
+		else traceBvh<kMetrics, kCap, kShort, kSlabIeee, kBatch>(S, R, ray, best, c, lds);
 	}
 #else
-	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort>(S, R, ray, best, c, lds);
+	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort, kSlabAny, kBatch>(S, R, ray, best, c, lds);
 #endif
 	else traceLinear<kMetrics>(S, R, ray, best, c);
 	c.add(2, (uint32_t) R.planeCount);
@@ -1304,14 +1303,14 @@ __device__ __forceinline__ bool shadeSegment(const DevScene &S, const SceneRange
 }
 
 // tracePath (GpuKernelSources.java:1061-1165).
-template <bool kMetrics, int kCap>
+template <bool kMetrics, int kCap, int kBatch = 0>
 __device__ inline f3 tracePath(const DevScene &S, const SceneRange &R, f3 o, f3 d, int maxDepth, int rrStart,
                                uint32_t &rng, Counters<kMetrics> &c, LdsStack lds) {
 	f3 T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
 	for (int depth = 0; depth < maxDepth; ++depth) {
 		c.add(0, 1);
 		Best best;
-		const bool found = traceClosest<kMetrics, kCap>(S, R, o, d, best, c, lds);
+		const bool found = traceClosest<kMetrics, kCap, kShortStack, kBatch>(S, R, o, d, best, c, lds);
 		if (!shadeSegment(S, R, found, best, depth, maxDepth, rrStart, o, d, T, L, rng)) break;
 	}
 	return L;

@@ -71,8 +71,9 @@ struct RenderParams {
 // renderKernel (GpuKernelSources.java:1167-1315).  One thread per pixel runs
 // samples [s0, s1) of the frame and keeps the running sum in registers; the
 // float sum order and the last-sample normalization are the reference's.
-// A wave covers an 8x8 pixel block for primary-ray coherence.
-template <bool kMetrics, int kCap>
+// A wave covers an 8x8 pixel block for primary-ray coherence.  kBatch > 0
+// tests popped leaves kBatch lanes at a time (traceBvh's leaf batching).
+template <bool kMetrics, int kCap, int kBatch = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_RENDER_WAVES_PER_EU)))
 void renderKernel(RenderParams P) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -93,7 +94,7 @@ void renderKernel(RenderParams P) {
 			uint32_t rng = path_seed(P.frameSeed, P.frameIndex, pixel, s);
 			f3 o, d;
 			cameraRay(P.cam, P.width, P.height, x, y, rng, o, d);
-			const f3 col = tracePath<kMetrics, kCap>(P.S, P.R, o, d, P.maxDepth, P.rrStart, rng, c, lds);
+			const f3 col = tracePath<kMetrics, kCap, kBatch>(P.S, P.R, o, d, P.maxDepth, P.rrStart, rng, c, lds);
 			acc = s == 0 ? col : acc + col;
 			if (s == P.total - 1) acc = acc * (1.0f / (float) P.total);
 		}
@@ -154,8 +155,12 @@ using RenderFn = void (*)(RenderParams);
 using ReplayFn = void (*)(ReplayParams);
 // Stack storage capacities compiled ahead of time; a limit up to 64 uses the
 // 64-entry storage (LDS top + private spill), larger limits the 256 one.
+// Leaf batching is compiled for one width, kLeafBatchLanes (16: the fastest
+// of 8 / 16 / 32 on the Layered Hero frames).
+constexpr int kLeafBatchLanes = 16;
 template <bool M>
-RenderFn renderFor(int cap) {
+RenderFn renderFor(int cap, int leafBatch) {
+	if (leafBatch > 0) return cap <= 64 ? renderKernel<M, 64, kLeafBatchLanes> : renderKernel<M, 256, kLeafBatchLanes>;
 	if (cap <= 64) return renderKernel<M, 64>;
 	return renderKernel<M, 256>;
 }
@@ -268,6 +273,16 @@ int mrad_set_russian_roulette(mrad_tracer *t, int startDepth) {
 	return MRAD_OK;
 }
 
+int mrad_set_leaf_batch(mrad_tracer *t, int lanes) {
+	clear_error();
+	if (!t) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle is null");
+	if (lanes != 0 && lanes != kLeafBatchLanes)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "leaf batch must be 0 or " + std::to_string(kLeafBatchLanes));
+	std::lock_guard<std::mutex> g(t->mu);
+	t->leafBatch = lanes;
+	return MRAD_OK;
+}
+
 // ensureSceneResident / uploadScene (GpuRayTracer.java:873-948).
 int mrad_upload_scene(mrad_tracer *t, const mrad_scene_arrays *a, uint64_t key) {
 	clear_error();
@@ -325,7 +340,8 @@ int mrad_render_sample(mrad_tracer *t, float *rgb, const float camera[12], int m
 	P.out = t->output.as<float>();
 	P.stats = metrics ? t->stats.as<unsigned long long>() : nullptr;
 	const int blocks = P.tilesX * ((H + kTile - 1) / kTile);
-	RenderFn fn = metrics ? renderFor<true>(t->cfg.bvh_stack_size) : renderFor<false>(t->cfg.bvh_stack_size);
+	RenderFn fn = metrics ? renderFor<true>(t->cfg.bvh_stack_size, t->leafBatch)
+	                      : renderFor<false>(t->cfg.bvh_stack_size, t->leafBatch);
 	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
 	for (int l = 0; l < launches; l++) {
 		P.s0 = l * per;

@@ -25,6 +25,7 @@ struct mrad_tracer {
 	mrad::DevBuf output, stats, rays, flags, dist, order, mat, normals, perRay;
 	mrad::DevBuf gradImage, gradAlbedo, gradEmission;
 	int rrStart = -1;
+	int leafBatch = 0;                  // megakernel leaf batching: 0 or kLeafBatchLanes
 	mrad_frame_stats last{};
 };
 

@@ -91,6 +91,7 @@ SIGNATURES = {
     "mrad_is_available": (C.c_int, [C.c_void_p]),
     "mrad_upload_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneArrays), C.c_uint64]),
     "mrad_set_russian_roulette": (C.c_int, [C.c_void_p, C.c_int]),
+    "mrad_set_leaf_batch": (C.c_int, [C.c_void_p, C.c_int]),
     "mrad_render_sample": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64]),
     "mrad_copy_frame": (C.c_int, [C.c_void_p, fp]),
     "mrad_trace_replay": (C.c_int, [C.c_void_p, fp, C.c_int, C.c_int, ip, fp, ip, ip, fp, up]),

@@ -144,6 +144,11 @@ class GpuRayTracer:
     def set_russian_roulette(self, start_depth: int) -> None:
         N.check(N.lib.mrad_set_russian_roulette(self._handle(), int(start_depth)))
 
+    def set_leaf_batch(self, lanes: int) -> None:
+        """0 (default) or 16: the megakernel tests popped leaves 16 lanes at a
+        time; same images and counters, faster on leaf-heavy scenes."""
+        N.check(N.lib.mrad_set_leaf_batch(self._handle(), int(lanes)))
+
     def ensure_scene_resident(self, scene: GpuScene) -> None:
         N.check(N.lib.mrad_upload_scene(self._handle(), C.byref(scene.arrays()), C.c_uint64(scene.key)))
 

@@ -78,6 +78,19 @@ def test_argument_checks_precede_device_checks():
     assert b"null" in N.lib.mrad_last_error()
 
 
+def test_leaf_batch_setter_checks_arguments():
+    """mrad_set_leaf_batch: 0 or 16 on any tracer handle (no device needed); a
+    null handle and other widths are invalid arguments."""
+    t = mrad.GpuRayTracer(4, 4, 1, True)
+    t.set_leaf_batch(16)
+    t.set_leaf_batch(0)
+    for bad in (-1, 1, 8, 32):
+        with pytest.raises(mrad.InvalidArgument):
+            t.set_leaf_batch(bad)
+    assert N.lib.mrad_set_leaf_batch(None, 0) == N.MRAD_ERR_INVALID_ARGUMENT
+    assert b"null" in N.lib.mrad_last_error()
+
+
 def test_destroy_null_is_noop():
     N.lib.mrad_destroy(None)
     N.lib.mrad_scene_destroy(None)

@@ -1,6 +1,6 @@
 """GPU: the reference's own Layered Hero metrics frames (1920x1080, 1 spp,
 depth 12, stack 32, frame seed BenchmarkProtocol.metricsSeed), rendered by
-the megakernel and by the wavefront form, against the oracle bit for bit:
+the megakernel (with and without leaf batching) and by the wavefront form, against the oracle bit for bit:
 every pixel's float bits and all 14 frame counters.  These scenes exercise
 the mirror, glass and emissive lobes and thousands of affine boxes."""
 from __future__ import annotations
@@ -30,14 +30,18 @@ def test_hero_metrics_frame_bit_exact(require_gpu, row, scale, lam):
     ref, ref_st = O.render_pixels(O.Scene(sc.wire(), stack_size=W.STACK_CAPACITY), cam.packed(), W.WIDTH, W.HEIGHT,
                                   W.MAXIMUM_PATH_DEPTH, 0, seed, 1)
     ref_stats = dict(zip(mrad.STAT_NAMES, map(int, ref_st)))
-    img = np.zeros(W.WIDTH * W.HEIGHT * 3, np.float32)
-    with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, W.SAMPLES_PER_PIXEL, True, collect_metrics=True,
-                           bvh_stack_size=W.STACK_CAPACITY) as t:
-        t.render_sample(img, sc, cam, W.MAXIMUM_PATH_DEPTH, 0, seed)
-        st = t.last_frame_stats()
-    np.testing.assert_array_equal(img.view(np.uint32), ref.reshape(-1).view(np.uint32))
-    for k in mrad.STAT_NAMES:
-        assert st[k] == ref_stats[k], k
+    # the megakernel without and with leaf batching (16 lanes), with and without counters
+    for lanes, metrics in ((0, True), (16, True), (16, False)):
+        img = np.zeros(W.WIDTH * W.HEIGHT * 3, np.float32)
+        with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, W.SAMPLES_PER_PIXEL, True, collect_metrics=metrics,
+                               bvh_stack_size=W.STACK_CAPACITY) as t:
+            t.set_leaf_batch(lanes)
+            t.render_sample(img, sc, cam, W.MAXIMUM_PATH_DEPTH, 0, seed)
+            st = t.last_frame_stats()
+        np.testing.assert_array_equal(img.view(np.uint32), ref.reshape(-1).view(np.uint32))
+        if metrics:
+            for k in mrad.STAT_NAMES:
+                assert st[k] == ref_stats[k], (lanes, k)
 
     # the same frame in the wavefront form, with and without counters
     for metrics in (True, False):
