@@ -82,10 +82,11 @@ def test_render_bit_exact(require_gpu, name, w, h, spp, depth, rr, frame):
     seed = mrad.frame_seed(0x1234ABCD, frame)
     print(f"render {name} {w}x{h} spp={spp} depth={depth} rr={rr}", flush=True)
     ref, ref_stats = O.render_pixels(O.Scene(scene.wire()), cam.packed(), w, h, depth, frame, seed, spp, rr_start=rr)
-    for metrics in (False, True):
+    for metrics, lanes in ((False, 0), (True, 0), (True, 16)):
         with mrad.GpuRayTracer(w, h, spp, True, collect_metrics=metrics) as t:
             if rr >= 0:
                 t.set_russian_roulette(rr)
+            t.set_leaf_batch(lanes)
             img = np.zeros(w * h * 3, np.float32)
             t.render_sample(img, scene, cam, depth, frame, seed)
             stats = t.last_frame_stats()
@@ -93,6 +94,25 @@ def test_render_bit_exact(require_gpu, name, w, h, spp, depth, rr, frame):
         if metrics:
             got = np.array([stats[k] for k in mrad.STAT_NAMES], dtype=np.uint64)
             np.testing.assert_array_equal(got, ref_stats)
+
+
+@pytest.mark.parametrize("lanes", [0, 16])
+def test_render_overflow_fallback_bit_exact(require_gpu, lanes):
+    """A capacity-8 stack overflows on the clustered field: the linear fallback
+    of traceBvh, in the plain and in the leaf-batched traversal loop."""
+    scene, cam = SCENES["clustered4k"][0](), mrad.GpuCamera.look_at((0.2, 0.1, -0.3), (0, 0, 1), 90)
+    w, h, spp, depth, frame = 160, 90, 2, 6, 3
+    seed = mrad.frame_seed(0x5EED, frame)
+    print(f"render clustered4k stack 8 leaf batch {lanes}", flush=True)
+    ref, ref_stats = O.render_pixels(O.Scene(scene.wire(), stack_size=8), cam.packed(), w, h, depth, frame, seed, spp)
+    assert int(ref_stats[mrad.STAT_NAMES.index("stack_overflows")]) > 0
+    with mrad.GpuRayTracer(w, h, spp, True, collect_metrics=True, bvh_stack_size=8) as t:
+        t.set_leaf_batch(lanes)
+        img = np.zeros(w * h * 3, np.float32)
+        t.render_sample(img, scene, cam, depth, frame, seed)
+        stats = t.last_frame_stats()
+    np.testing.assert_array_equal(img.view(np.uint32), ref.reshape(-1).view(np.uint32))
+    np.testing.assert_array_equal(np.array([stats[k] for k in mrad.STAT_NAMES], dtype=np.uint64), ref_stats)
 
 
 def test_render_split_launches_equal(require_gpu):
