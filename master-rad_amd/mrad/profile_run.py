@@ -4,6 +4,8 @@
     python -m mrad.profile_run batch      # the 64-scene batch, one wavefront render
     python -m mrad.profile_run copy       # one device copy of 1 GiB (calibrates FETCH_SIZE / WRITE_SIZE)
     python -m mrad.profile_run adjoint    # diff-10k forward + backward
+    python -m mrad.profile_run hero       # Layered Hero 1M weighted, 1080p, 1 spp, depth 12: 5 megakernel
+                                          # frames with leaf batching (16 lanes)
 
 Run from master-rad_amd/ (or with it on sys.path).  Each prints one line per
 step so a profiled run shows progress.
@@ -70,6 +72,19 @@ def _adjoint():
     r.close()
 
 
+def _hero(frames: int = 5):
+    import mrad
+    from mrad import workloads as W
+    sc = W.hero_scene(1_000_000, mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, 1.0))
+    cam = W.hero_gpu_camera()
+    print("hero-1m built", flush=True)
+    with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
+        t.set_leaf_batch(16)
+        for i in range(frames):
+            t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(7, i))
+            print(f"hero-1m frame {i}: {t.last_frame_stats()['kernel_nanos'] / 1e6:.3f} ms", flush=True)
+
+
 def main(argv):
     what = argv[1] if len(argv) > 1 else "het"
     if what == "het":
@@ -80,6 +95,8 @@ def main(argv):
         _copy()
     elif what == "adjoint":
         _adjoint()
+    elif what == "hero":
+        _hero()
     else:
         raise SystemExit(f"unknown workload {what}")
 
