@@ -918,13 +918,18 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 					int nearN, farN;
 					float nearE, farE;
 					orderChildren(nd, le, re, nearN, farN, nearE, farE);
-					if (isfinite(farE)) {
-						if (size >= cap) overflow = true;
-						else { b.st.set(size, farN, farE); size++; c.maxStack(size); }
-					}
-					if (!overflow && isfinite(nearE)) {
-						if (size >= cap) overflow = true;
-						else { b.st.set(size, nearN, nearE); size++; c.maxStack(size); }
+					// the default loop's branch-free push (same contents, maximum
+					// and overflow point)
+					const int s1 = size + (isfinite(farE) ? 1 : 0);
+					const int s2 = s1 + (isfinite(nearE) ? 1 : 0);
+					if (s2 > cap) {
+						c.maxStack(cap);
+						overflow = true;
+					} else {
+						b.st.set(size, farN, farE);
+						b.st.set(s1, nearN, nearE);
+						size = s2;
+						c.maxStack(size);
 					}
 				}
 			}
