@@ -157,7 +157,10 @@ using ReplayFn = void (*)(ReplayParams);
 // 64-entry storage (LDS top + private spill), larger limits the 256 one.
 // Leaf batching is compiled for one width, kLeafBatchLanes (16: the fastest
 // of 8 / 16 / 32 on the Layered Hero frames).
-constexpr int kLeafBatchLanes = 16;
+#ifndef MRAD_LEAF_BATCH_LANES
+#define MRAD_LEAF_BATCH_LANES 16
+#endif
+constexpr int kLeafBatchLanes = MRAD_LEAF_BATCH_LANES;
 template <bool M>
 RenderFn renderFor(int cap, int leafBatch) {
 	if (leafBatch > 0) return cap <= 64 ? renderKernel<M, 64, kLeafBatchLanes> : renderKernel<M, 256, kLeafBatchLanes>;
