@@ -1,0 +1,108 @@
+"""Command-line driver (CPU): the -Dgfxlab.* property checks of
+BenchmarkRunner (playground/BenchmarkRunner.java:18-41, 86-188), Long.decode
+of the seed, the configuration string, the file name and the BVH build
+configuration read from properties (gpu/BvhBuildConfig.java:52-64)."""
+from __future__ import annotations
+
+import datetime as dt
+
+import pytest
+
+import mrad
+from mrad import benchmark_runner as B
+
+BASE = {"gfxlab.frames": "3", "gfxlab.warmupFrames": "1", "gfxlab.width": "64", "gfxlab.height": "32",
+        "gfxlab.maxDepth": "12", "gfxlab.progressEvery": "1", "gfxlab.seed": "0x5eed", "gfxlab.bvhLeafSize": "8",
+        "gfxlab.benchmarkProcessId": "2", "gfxlab.benchmarkPhase": "timing", "gfxlab.gpu.samplesPerFrame": "4",
+        "gfxlab.versions": "gpu_bvh,GPU_WEIGHTED_BVH", "gfxlab.scenes": "hero-96,OVERLAP_10K",
+        "gfxlab.benchmarkDir": "runs"}
+
+
+def _argv(props):
+    return [f"-D{k}={v}" for k, v in props.items()]
+
+
+def test_plan_from_properties():
+    plan = B.Plan(B.parse_properties(_argv(BASE)))
+    assert (plan.frames, plan.warmup_frames, plan.width, plan.height, plan.max_depth) == (3, 1, 64, 32, 12)
+    assert plan.seed == 0x5EED and plan.versions == ["GPU_BVH", "GPU_WEIGHTED_BVH"]
+    assert plan.scenes == ["hero-96", "overlap-10k"] and plan.stack_size == 32 and plan.leaf_batch == 0
+    assert not plan.collect_metrics
+    text = plan.configuration()
+    assert text.startswith("phase:timing,processId:2,width:64,height:32,maxDepth:12,bvhLeafSize:8,"
+                           "warmupFrames:1,measuredFrames:3,seed:0x5eed,python:")
+    assert text.endswith(",logicalSamplesPerFrame:4,renderPixelsPerLaunch:2048,replayRaysPerLaunch:2048,"
+                         "samplesPerPhysicalKernel:1,physicalKernelLaunchesPerLogicalFrame:1,"
+                         "kernelNanosSemantics:" + B.KERNEL_NANOS_SEMANTICS)
+    assert plan.configs["GPU_BVH"] == mrad.BvhBuildConfig.uniform(8)
+    assert plan.configs["GPU_WEIGHTED_BVH"] == mrad.BvhBuildConfig.weighted(8, 1.0)
+
+
+@pytest.mark.parametrize("key,value,message", [
+    ("gfxlab.frames", "0", "must be positive"),
+    ("gfxlab.frames", "2.5", "must be an integer"),
+    ("gfxlab.frames", "2147483648", "must be an integer"),
+    ("gfxlab.warmupFrames", "-1", "must be nonnegative"),
+    ("gfxlab.width", "", "Missing required system property gfxlab.width"),
+    ("gfxlab.seed", "0x", "signed 64-bit"),
+    ("gfxlab.seed", "08", "signed 64-bit"),
+    ("gfxlab.seed", "0x8000000000000000", "signed 64-bit"),
+    ("gfxlab.benchmarkPhase", "warm", "timing or counters"),
+    ("gfxlab.versions", "gpu_bvh,,gpu_linear", "empty, malformed, or repeated"),
+    ("gfxlab.versions", "gpu_bvh,gpu_bvh", "empty, malformed, or repeated"),
+    ("gfxlab.versions", "cpu_bvh", "runs on the CPU"),
+    ("gfxlab.versions", "gpu_fast", "Unknown renderer version"),
+    ("gfxlab.scenes", "gi_test", "Unknown scene preset"),
+    ("gfxlab.scenes", "hero-1k,HERO_1K", "empty, malformed, or repeated"),
+    ("gfxlab.gpu.samplesPerFrame", "0", "must be positive"),
+    ("gfxlab.gpu.bvhStackSize", "4", "between 8 and 256"),
+    ("gfxlab.gpu.leafBatch", "8", "0 or 16"),
+    ("gfxlab.bvh.lambda", "-1", "nonnegative"),
+    ("gfxlab.bvh.boxWeight", "0", "positive"),
+    ("gfxlab.benchmarkDir", " ", "Missing required system property"),
+])
+def test_property_errors(key, value, message):
+    props = dict(BASE, **{key: value})
+    with pytest.raises(mrad.InvalidArgument, match=message):
+        B.Plan(B.parse_properties(_argv(props)))
+
+
+def test_missing_property_and_positional_argument():
+    props = dict(BASE)
+    del props["gfxlab.maxDepth"]
+    with pytest.raises(mrad.InvalidArgument, match="gfxlab.maxDepth"):
+        B.Plan(B.parse_properties(_argv(props)))
+    with pytest.raises(mrad.InvalidArgument, match="no positional arguments"):
+        B.parse_properties(["hero-96"])
+
+
+@pytest.mark.parametrize("raw,value", [("0", 0), ("42", 42), ("-42", -42), ("+7", 7), ("0x5EED", 0x5EED),
+                                       ("#ff", 255), ("017", 15), ("-0x8000000000000000", -(1 << 63)),
+                                       ("9223372036854775807", (1 << 63) - 1)])
+def test_seed_long_decode(raw, value):
+    assert B.required_long({"gfxlab.seed": raw}, "gfxlab.seed") == value
+
+
+def test_negative_seed_prints_twos_complement():
+    text = B.base_configuration("counters", 1, 8, 8, 4, 8, 0, 1, -1)
+    assert ",seed:0xffffffffffffffff," in text
+
+
+def test_weighted_config_from_properties_and_unweighted_lambda():
+    props = dict(BASE, **{"gfxlab.bvh.lambda": "0.5", "gfxlab.bvh.boxWeight": "2.0", "gfxlab.bvhLeafSize": "4"})
+    plan = B.Plan(B.parse_properties(_argv(props)))
+    w, u = plan.configs["GPU_WEIGHTED_BVH"], plan.configs["GPU_BVH"]
+    assert (w.mode, w.leaf_size, w.lam, w.box_weight) == ("WEIGHTED_SAH", 4, 0.5, 2.0)
+    assert (u.mode, u.lam, u.box_weight) == ("UNIFORM_SAH", 0.0, 2.0)
+    assert w.summary() == ("mode=WEIGHTED_SAH,leafSize=4,interiorTraversal=1.275577,sphere=1.000000,"
+                           "box=2.000000,affineSphere=2.552744,affineBox=2.728115,lambda=0.500000")
+
+
+def test_benchmark_file_name():
+    now = dt.datetime(2026, 3, 4, 5, 6, 7, 89_123)
+    assert B.benchmark_file("runs", "hero-1m", "GPU_WEIGHTED_BVH", now).as_posix() == \
+        "runs/hero-1m-gpu_weighted_bvh-20260304-050607-089.csv"
+
+
+def test_boolean_property():
+    assert B.java_boolean({"k": "TRUE"}, "k") and not B.java_boolean({"k": "1"}, "k") and not B.java_boolean({}, "k")

@@ -57,3 +57,28 @@ def test_headless_renderer_writes_reference_rows(require_gpu, tmp_path):
     assert int(row["rootAabbTests"]) == ref["aabb_tests"] - 2 * ref["internal_node_visits"]
     assert row["traversalWork"] == R.java_fixed(R.traversal_work_of(ref), 4)
     assert row["uploadMs"] == "0.0000" and float(row["kernelMs"]) > 0.0
+
+
+def test_benchmark_runner_writes_one_file_per_pair(require_gpu, tmp_path, capsys):
+    """The command-line driver over two versions and one scene, counters on:
+    one CSV per pair with a row per measured frame, exit status 0."""
+    from mrad import benchmark_runner as B
+    props = {"gfxlab.frames": "2", "gfxlab.warmupFrames": "1", "gfxlab.width": "64", "gfxlab.height": "32",
+             "gfxlab.maxDepth": "12", "gfxlab.progressEvery": "1", "gfxlab.seed": "7", "gfxlab.bvhLeafSize": "8",
+             "gfxlab.benchmarkProcessId": "1", "gfxlab.benchmarkPhase": "counters",
+             "gfxlab.gpu.samplesPerFrame": "2", "gfxlab.gpu.collectMetrics": "true", "gfxlab.gpu.leafBatch": "16",
+             "gfxlab.versions": "gpu_linear,gpu_weighted_bvh", "gfxlab.scenes": "hero-96",
+             "gfxlab.benchmarkDir": str(tmp_path)}
+    print("benchmark runner: hero-96 64x32, GPU_LINEAR and GPU_WEIGHTED_BVH", flush=True)
+    assert B.main([f"-D{k}={v}" for k, v in props.items()]) == 0
+    files = sorted(tmp_path.glob("*.csv"))
+    assert [f.name.rsplit("-", 3)[0] for f in files] == ["hero-96-gpu_linear", "hero-96-gpu_weighted_bvh"]
+    for f, mode in zip(files, ("mode=LINEAR", "mode=WEIGHTED_SAH")):
+        lines = f.read_text().splitlines()
+        assert f"accelerationConfig={mode}" in lines[0] and "config=phase:counters," in lines[0]
+        assert lines[1] == ",".join(R.COLUMNS) and len(lines) == 4
+        rows = [dict(zip(R.COLUMNS, line.split(","))) for line in lines[2:]]
+        assert all(r["count"] == "2" and int(r["cameraSamples"]) == 64 * 32 * 2 for r in rows)
+    linear, bvh = ([dict(zip(R.COLUMNS, f.read_text().splitlines()[3].split(",")))] for f in files)
+    assert int(linear[0]["aabbTests"]) == 0 and int(bvh[0]["aabbTests"]) > 0
+    assert "=== Running GPU_WEIGHTED_BVH on scene hero-96" in capsys.readouterr().out
