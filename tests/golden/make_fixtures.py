@@ -88,10 +88,13 @@ def bvh_matrix():
         with open(STUDY / name) as f:
             tables[key] = {r["rowId"]: r for r in csv.DictReader(f)}
     rows = []
-    for row_id in list(SCALES) + ["overlap-10k"]:
+    random_rows = [f"F{f}-{n}" for f in range(1, 5) for n in ("100", "1k", "10k")]
+    for row_id in list(SCALES) + ["overlap-10k"] + random_rows:
         for method in ("uniform", "weighted"):
+            # random rows hold medians over 100 layouts (a half-integer when the middle pair differs)
             rows.append({"rowId": row_id, "method": method,
-                         **{k: int(t[row_id][method]) for k, t in tables.items()}})
+                         **{k: float(t[row_id][method]) if row_id in random_rows else int(t[row_id][method])
+                            for k, t in tables.items()}})
     (OUT / "bvh_matrix.json").write_text(json.dumps(rows, indent=0) + "\n")
 
 
