@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -368,6 +369,76 @@ def bench_single(args, w: configs.Workload) -> dict:
     }, [w]
 
 
+def bench_random(args) -> list:
+    """The random first-hit rows F1-F4 at 100 / 1k / 10k objects
+    (RandomSingleRaySceneFactory, the study's look-at camera; 1920x1080, 1 spp,
+    depth 12, megakernel without leaf batching), over the first
+    ``--random-layouts`` layouts of each row.  Per layout, as the reference's
+    analysis does (benchmarks/analyze_results.py:669-706, 1132-1140): the work
+    reduction of weighted against uniform SAH from one counter frame each, and
+    the time reduction as the median of four ABBA blocks, each
+    100 (1 - sqrt(B1 B2 / (A1 A2))) over kernel times.  A row reports the
+    medians over its layouts beside the published ones (RTX 3080, 100 layouts)."""
+    from mrad import random_scenes as RS
+    from mrad import workloads as W
+    from mrad.recorder import traversal_work_of
+    pub = {r["rowId"]: r for r in json.loads((ROOT / "tests" / "golden" / "random_reductions.json").read_text())}
+    cam = RS.random_gpu_camera(W.WIDTH, W.HEIGHT)
+    kw = dict(bvh_stack_size=W.STACK_CAPACITY)
+    rows = []
+    with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, collect_metrics=True, **kw) as tm, \
+            mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, **kw) as ta, \
+            mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, **kw) as tb:
+        for count in RS.OBJECT_COUNTS:
+            cache: dict = {}
+            for fam in RS.FAMILIES:
+                rid = RS.row_id(fam, count)
+                t_red, w_red, rays, kms = [], [], [], {"uniform": [], "weighted": []}
+                for lid in range(args.random_layouts):
+                    scenes, work = {}, {}
+                    for method, cfg in (("uniform", mrad.BvhBuildConfig.uniform(W.LEAF_SIZE)),
+                                        ("weighted", mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, 1.0))):
+                        scenes[method] = RS.random_scene(fam, count, lid, cfg, cache)
+                        tm.render_sample(None, scenes[method], cam, W.MAXIMUM_PATH_DEPTH, 0, RS.metrics_seed(rid, lid))
+                        st = tm.last_frame_stats()
+                        work[method] = traversal_work_of(st)
+                        if method == "uniform":
+                            rays.append(st["rays"])
+                    w_red.append(100.0 * (1.0 - work["weighted"] / work["uniform"]))
+                    base = W.timing_seed(RS.workload_key(rid, lid), 0)
+
+                    def frame(t, method, i):
+                        t.render_sample(None, scenes[method], cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(base, i))
+                        ns = t.last_frame_stats()["kernel_nanos"]
+                        kms[method].append(ns / 1e6)
+                        return ns
+
+                    for t, method in ((ta, "uniform"), (tb, "weighted")):          # uploads and warm-up
+                        t.render_sample(None, scenes[method], cam, W.MAXIMUM_PATH_DEPTH, 0, mrad.frame_seed(base, 0))
+                    blocks = []
+                    for blk in range(4):
+                        s1, s2 = 1 + 2 * blk, 2 + 2 * blk
+                        a1, b1, b2, a2 = frame(ta, "uniform", s1), frame(tb, "weighted", s1), \
+                            frame(tb, "weighted", s2), frame(ta, "uniform", s2)
+                        blocks.append(100.0 * (1.0 - math.sqrt(b1 * b2 / (a1 * a2))))
+                    t_red.append(float(np.median(blocks)))
+                row = {"row": rid, "primitives": count, "layouts": args.random_layouts,
+                       "uniform_kernel_ms": float(np.median(kms["uniform"])),
+                       "weighted_kernel_ms": float(np.median(kms["weighted"])),
+                       "weighted_time_reduction_pct": float(np.median(t_red)),
+                       "weighted_work_reduction_pct": float(np.median(w_red)),
+                       "median_segments": float(np.median(rays)),
+                       "published_rtx3080_time_reduction_pct": pub[rid]["timeReductionPercent"],
+                       "published_work_reduction_pct": pub[rid]["workReductionPercent"],
+                       "published_median_segments": pub[rid]["medianRays"]}
+                log(f"{rid}: {row['uniform_kernel_ms']:.3f} / {row['weighted_kernel_ms']:.3f} ms kernel, reduction: "
+                    f"time {row['weighted_time_reduction_pct']:+.2f} % (published {pub[rid]['timeReductionPercent']:+.2f} %), "
+                    f"work {row['weighted_work_reduction_pct']:+.2f} % (published {pub[rid]['workReductionPercent']:+.2f} %), "
+                    f"median segments {row['median_segments']:.0f} (published {pub[rid]['medianRays']:.0f})")
+                rows.append(row)
+    return rows
+
+
 def bench_hero(args) -> dict:
     """The reference's own Layered Hero timing frames (LayeredHeroFamily, 96 to
     1M primitives; 1920x1080, 1 spp, depth 12, stack 32) in the megakernel,
@@ -451,6 +522,7 @@ def bench_hero(args) -> dict:
                  "weighted_kernel_ms_leaf_batch": olb["weighted"],
                  "weighted_time_reduction_pct": 100.0 * (1.0 - oms["weighted"] / oms["uniform"]),
                  "published_rtx3080_time_reduction_pct": ovl[("overlap-10k", "weighted")]["timeReductionPercent"]})
+    rows.extend(bench_random(args))
     paths = W.WIDTH * W.HEIGHT
     return {
         "metric": METRIC, "value": paths / (line["wall_ms"] / 1e3) / 1e6, "unit": "Mpaths/s", "n_gpus": 1,
@@ -478,6 +550,8 @@ def main():
     ap.add_argument("--workload", choices=["batch", "het", "clustered", "hero"], default="batch")
     ap.add_argument("--queue-capacity", type=int, default=1 << 26)
     ap.add_argument("--waves-per-cu", type=int, default=28)
+    ap.add_argument("--random-layouts", type=int, default=10,
+                    help="layouts per random F1-F4 row timed by --workload hero (the study used 100)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="tune queue capacity, waves per CU and the LDS split on the batch before timing")

@@ -110,6 +110,19 @@ def modeled_work():
     (OUT / "modeled_work.json").write_text(json.dumps(rows, indent=0) + "\n")
 
 
+def random_reductions():
+    """Weighted-vs-uniform medians over the 100 layouts of each random row (core8-detail.csv)."""
+    rows = []
+    with open(STUDY / "core8-detail.csv") as f:
+        for r in csv.DictReader(f):
+            if r["rowId"].startswith("F") and r["methodId"] == "weighted":
+                rows.append({"rowId": r["rowId"], "layoutCount": int(r["layoutCount"]),
+                             "timeReductionPercent": float(r["timeReductionPercent"]),
+                             "workReductionPercent": float(r["workReductionPercent"]),
+                             "medianRays": float(r["medianRays"]), "medianModeledWork": float(r["medianModeledWork"])})
+    (OUT / "random_reductions.json").write_text(json.dumps(rows, indent=0) + "\n")
+
+
 if __name__ == "__main__":
     gi_test_topology()
     hero_topology()
@@ -117,3 +130,4 @@ if __name__ == "__main__":
     overlap_rows()
     bvh_matrix()
     modeled_work()
+    random_reductions()
