@@ -15,15 +15,17 @@ Renderer versions are the GPU ones of RendererFactory.Version
 GPU_WEIGHTED_BVH (weighted SAH, BvhBuildConfig.fromProperties with the
 ``gfxlab.bvh.*`` weights, gpu/BvhBuildConfig.java:52-64).  CPU_LINEAR and
 CPU_BVH are rejected: this package renders on the GPU only.  The scenes are
-the benchmark study's workloads (hero-96, hero-1k, hero-10k, hero-100k,
-hero-1m and overlap-10k) with their published cameras, in place of the
+the benchmark study's workloads (hero-96 to hero-1m, overlap-10k and the
+random rows f1-100 to f4-10k) with their published cameras, in place of the
 interactive SceneCatalog presets.
 
 ``gfxlab.gpu.collectMetrics=true`` turns on the traversal counters (the
 reference's counters phase), ``gfxlab.gpu.bvhStackSize`` sets the traversal
 stack (8..256, GpuRayTracer.java:1100-1103) and ``gfxlab.gpu.leafBatch`` (0 or
-16) the megakernel's leaf-batched traversal.  Every frame is one launch over the
-whole image, so the launch fields of the configuration string describe that.
+16) the megakernel's leaf-batched traversal.  The random rows (f1-100 to f4-10k)
+render the layout that ``gfxlab.layoutId`` names (0..99, default 0).  Every
+frame is one launch over the whole image, so the launch fields of the
+configuration string describe that.
 """
 from __future__ import annotations
 
@@ -42,7 +44,8 @@ from .tracer import HeadlessGpuRenderer
 BENCHMARK_FLUSH_EVERY = 20                   # BenchmarkRunner.java:13
 GPU_VERSIONS = ("GPU_LINEAR", "GPU_BVH", "GPU_WEIGHTED_BVH")
 CPU_VERSIONS = ("CPU_LINEAR", "CPU_BVH")
-SCENES = ("hero-96", "hero-1k", "hero-10k", "hero-100k", "hero-1m", "overlap-10k")
+SCENES = ("hero-96", "hero-1k", "hero-10k", "hero-100k", "hero-1m", "overlap-10k",
+          *(f"{f.lower()}-{n}" for f in ("F1", "F2", "F3", "F4") for n in ("100", "1k", "10k")))
 SAMPLES_PER_PHYSICAL_KERNEL = 1
 KERNEL_NANOS_SEMANTICS = "one HIP-event interval around the complete physical-launch sequence of a logical frame"
 _INT = re.compile(r"[+-]?[0-9]+")
@@ -210,9 +213,15 @@ def benchmark_file(directory: str, scene: str, version: str, now: _dt.datetime |
     return Path(directory) / f"{scene}-{version.lower()}-{stamp}.csv"
 
 
-def scene_setup(name: str, config: BvhBuildConfig, width: int, height: int):
-    """The scene and camera of one benchmark workload."""
+def scene_setup(name: str, config: BvhBuildConfig, width: int, height: int, layout_id: int = 0):
+    """The scene and camera of one benchmark workload; a random row (f1-100 ..
+    f4-10k) takes its layout identifier."""
     from . import workloads as W
+    if name.startswith("f"):
+        from . import random_scenes as RS
+        family, size = name.upper().split("-")
+        count = {"100": 100, "1K": 1_000, "10K": 10_000}[size]
+        return RS.random_scene(family, count, layout_id, config), RS.random_gpu_camera(width, height)
     if name == "overlap-10k":
         return W.overlap_scene(config), W.gpu_camera(W.overlap_camera(), width, height)
     scale = {"hero-96": 96, "hero-1k": 1_000, "hero-10k": 10_000, "hero-100k": 100_000, "hero-1m": 1_000_000}[name]
@@ -262,11 +271,15 @@ class Plan:
         self.leaf_batch = optional_int(props, "gfxlab.gpu.leafBatch", 0)
         if self.leaf_batch not in (0, 16):
             raise _invalid("gfxlab.gpu.leafBatch must be 0 or 16")
+        self.layout_id = optional_int(props, "gfxlab.layoutId", 0)
+        if not 0 <= self.layout_id <= 99:
+            raise _invalid("gfxlab.layoutId must be between 0 and 99")
         self.directory = required(props, "gfxlab.benchmarkDir")
         self.configs = {v: bvh_config(props, v, self.leaf_size) for v in self.versions}
 
-    def configuration(self) -> str:
-        return self.base + gpu_configuration(self.width, self.height, self.samples_per_frame)
+    def configuration(self, scene_name: str = "") -> str:
+        text = self.base + gpu_configuration(self.width, self.height, self.samples_per_frame)
+        return text + f",layoutId:{self.layout_id}" if scene_name.startswith("f") else text
 
 
 def run(plan: Plan, out=None, err=None) -> int:
@@ -281,10 +294,11 @@ def run(plan: Plan, out=None, err=None) -> int:
             print(f"=== Running {version} on scene {scene_name} ({plan.warmup_frames} warm-up + {plan.frames} "
                   f"measured frames, {plan.width}x{plan.height}) ===", file=out, flush=True)
             try:
-                scene, camera = scene_setup(scene_name, plan.configs[version], plan.width, plan.height)
+                scene, camera = scene_setup(scene_name, plan.configs[version], plan.width, plan.height,
+                                            plan.layout_id)
                 path = benchmark_file(plan.directory, scene_name, version)
                 with BenchmarkRecorder(path, BENCHMARK_FLUSH_EVERY, cpu, gpu, version.lower(), scene_name,
-                                       plan.configuration()) as recorder:
+                                       plan.configuration(scene_name)) as recorder:
                     renderer = HeadlessGpuRenderer(scene, camera, plan.width, plan.height, plan.max_depth,
                                                    plan.seed, plan.samples_per_frame, version != "GPU_LINEAR",
                                                    collect_metrics=plan.collect_metrics,

@@ -106,3 +106,15 @@ def test_benchmark_file_name():
 
 def test_boolean_property():
     assert B.java_boolean({"k": "TRUE"}, "k") and not B.java_boolean({"k": "1"}, "k") and not B.java_boolean({}, "k")
+
+
+def test_random_rows_and_layout_id():
+    props = dict(BASE, **{"gfxlab.scenes": "F2-100,f4-1k", "gfxlab.layoutId": "7"})
+    plan = B.Plan(B.parse_properties(_argv(props)))
+    assert plan.scenes == ["f2-100", "f4-1k"] and plan.layout_id == 7
+    assert plan.configuration("f2-100").endswith(",layoutId:7") and "layoutId" not in plan.configuration("hero-96")
+    sc, cam = B.scene_setup("f2-100", mrad.BvhBuildConfig.uniform(8), 64, 32, 7)
+    c = sc.counts()
+    assert (c["spheres"], c["boxes"], c["affine_spheres"], c["affine_boxes"]) == (25, 25, 25, 25)
+    with pytest.raises(mrad.InvalidArgument, match="layoutId"):
+        B.Plan(B.parse_properties(_argv(dict(BASE, **{"gfxlab.layoutId": "100"}))))
