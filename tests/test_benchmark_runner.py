@@ -118,3 +118,16 @@ def test_random_rows_and_layout_id():
     assert (c["spheres"], c["boxes"], c["affine_spheres"], c["affine_boxes"]) == (25, 25, 25, 25)
     with pytest.raises(mrad.InvalidArgument, match="layoutId"):
         B.Plan(B.parse_properties(_argv(dict(BASE, **{"gfxlab.layoutId": "100"}))))
+
+
+def test_failed_pairs_exit_with_status_2(tmp_path, capsys):
+    """Without a usable device every (version, scene) pair fails with a message
+    on stderr and the run ends with status 2 (BenchmarkRunner.java:71-84)."""
+    with mrad.GpuRayTracer(1, 1, 1, True) as t:
+        if t.is_available():
+            pytest.skip("a GPU is present; the failure path is exercised on the CPU")
+    props = dict(BASE, **{"gfxlab.versions": "gpu_bvh", "gfxlab.scenes": "hero-96", "gfxlab.benchmarkDir": str(tmp_path)})
+    assert B.main(_argv(props)) == 2
+    captured = capsys.readouterr()
+    assert "=== Running GPU_BVH on scene hero-96" in captured.out
+    assert "Failed to run GPU_BVH on hero-96:" in captured.err
