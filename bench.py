@@ -557,7 +557,7 @@ def main():
                     help="tune queue capacity, waves per CU and the LDS split on the batch before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: every frame and batch renders on one GPU and this build has no "
+        print("bench.py measures one MI355X: every frame and batch renders on one GPU and the package has no "
               "multi-GPU form; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":

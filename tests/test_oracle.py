@@ -276,7 +276,8 @@ def test_emitter_seen_directly():
 
 
 def test_fp32_render_agrees_with_fp64_within_monte_carlo_band():
-    """north_star (c) on the room at 32x32: FP32 estimate vs FP64 reference, per-pixel 3.29 sigma band."""
+    """The FP32 path's tolerance on the room at 32x32: each pixel of the FP32
+    estimate lies within 3.29 sigma of the FP64 reference's mean."""
     w = configs.room()
     osc = O.Scene(w.scene.wire())
     cam = w.camera.packed()
