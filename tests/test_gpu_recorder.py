@@ -82,3 +82,28 @@ def test_benchmark_runner_writes_one_file_per_pair(require_gpu, tmp_path, capsys
     linear, bvh = ([dict(zip(R.COLUMNS, f.read_text().splitlines()[3].split(",")))] for f in files)
     assert int(linear[0]["aabbTests"]) == 0 and int(bvh[0]["aabbTests"]) > 0
     assert "=== Running GPU_WEIGHTED_BVH on scene hero-96" in capsys.readouterr().out
+
+
+def test_benchmark_runner_random_row(require_gpu, tmp_path):
+    """A random row through the driver: the layout named by gfxlab.layoutId,
+    its identifier in the configuration, and the frame's counters equal the
+    oracle's for that layout and frame seed."""
+    from mrad import benchmark_runner as B
+    from mrad import random_scenes as RS
+    props = {"gfxlab.frames": "1", "gfxlab.warmupFrames": "0", "gfxlab.width": "96", "gfxlab.height": "54",
+             "gfxlab.maxDepth": "12", "gfxlab.progressEvery": "1", "gfxlab.seed": "11", "gfxlab.bvhLeafSize": "8",
+             "gfxlab.benchmarkProcessId": "1", "gfxlab.benchmarkPhase": "counters",
+             "gfxlab.gpu.samplesPerFrame": "1", "gfxlab.gpu.collectMetrics": "true", "gfxlab.layoutId": "3",
+             "gfxlab.versions": "gpu_bvh", "gfxlab.scenes": "f2-100", "gfxlab.benchmarkDir": str(tmp_path)}
+    print("benchmark runner: f2-100 layout 3, 96x54", flush=True)
+    assert B.main([f"-D{k}={v}" for k, v in props.items()]) == 0
+    (f,) = tmp_path.glob("f2-100-gpu_bvh-*.csv")
+    lines = f.read_text().splitlines()
+    assert ",layoutId:3" in lines[0] and len(lines) == 3
+    row = dict(zip(R.COLUMNS, lines[2].split(",")))
+    sc = RS.random_scene("F2", 100, 3, mrad.BvhBuildConfig.uniform(8))
+    _, ref = O.render_pixels(O.Scene(sc.wire(), stack_size=32), RS.random_gpu_camera(96, 54).packed(), 96, 54,
+                             12, 0, mrad.frame_seed(11, 0), 1)
+    ref = dict(zip(mrad.STAT_NAMES, map(int, ref)))
+    assert int(row["pathSegments"]) == ref["rays"] and int(row["aabbTests"]) == ref["aabb_tests"]
+    assert int(row["affineSphereTests"]) == ref["affine_sphere_tests"] > 0
