@@ -6,6 +6,7 @@
     python -m mrad.profile_run adjoint    # diff-10k forward + backward
     python -m mrad.profile_run hero       # Layered Hero 1M weighted, 1080p, 1 spp, depth 12: 5 megakernel
                                           # frames with leaf batching (16 lanes)
+    python -m mrad.profile_run random     # F4-10k layout 0 weighted, 1080p, 1 spp, depth 12: 5 megakernel frames
 
 Run from master-rad_amd/ (or with it on sys.path).  Each prints one line per
 step so a profiled run shows progress.
@@ -85,6 +86,20 @@ def _hero(frames: int = 5):
             print(f"hero-1m frame {i}: {t.last_frame_stats()['kernel_nanos'] / 1e6:.3f} ms", flush=True)
 
 
+def _random(frames: int = 5):
+    """F4-10k layout 0 (10,000 affine boxes), weighted SAH, 1920x1080, 1 spp,
+    depth 12, stack 32: megakernel frames without leaf batching."""
+    import mrad
+    from mrad import random_scenes as RS
+    from mrad import workloads as W
+    sc = RS.random_scene("F4", 10_000, 0, mrad.BvhBuildConfig.weighted(W.LEAF_SIZE, 1.0))
+    cam = RS.random_gpu_camera(W.WIDTH, W.HEIGHT)
+    with mrad.GpuRayTracer(W.WIDTH, W.HEIGHT, 1, True, bvh_stack_size=W.STACK_CAPACITY) as t:
+        for i in range(frames):
+            t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(7, i))
+            print(f"F4-10k frame {i}: {t.last_frame_stats()['kernel_nanos'] / 1e6:.3f} ms", flush=True)
+
+
 def main(argv):
     what = argv[1] if len(argv) > 1 else "het"
     if what == "het":
@@ -97,6 +112,8 @@ def main(argv):
         _adjoint()
     elif what == "hero":
         _hero()
+    elif what == "random":
+        _random()
     else:
         raise SystemExit(f"unknown workload {what}")
 
