@@ -50,6 +50,8 @@ def _lib() -> C.CDLL:
     lib.or_sincosf.argtypes = [C.c_float, fp, fp]
     lib.or_path_seed.restype = C.c_uint32
     lib.or_path_seed.argtypes = [C.c_uint64, C.c_int, C.c_uint32, C.c_int]
+    lib.or_camera_ray.restype = None
+    lib.or_camera_ray.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32), fp]
     lib.or_render_pixels.restype = C.c_int
     lib.or_render_pixels.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_uint64, C.c_int, ip, C.c_int, fp, C.POINTER(C.c_uint64), C.c_int]
@@ -117,6 +119,15 @@ def sincosf(x: float) -> tuple[float, float]:
 
 def path_seed(frame_seed: int, frame_index: int, pixel: int, sample: int) -> int:
     return lib.or_path_seed(frame_seed, frame_index, pixel, sample)
+
+
+def camera_ray(camera: np.ndarray, width: int, height: int, x: int, y: int, rng: int):
+    """renderKernel's camera ray of pixel (x, y) from RNG state rng: (ray6 float32, rng after the two jitter draws)."""
+    cam = np.ascontiguousarray(camera, np.float32)
+    state = C.c_uint32(rng & 0xFFFFFFFF)
+    out = np.zeros(6, np.float32)
+    lib.or_camera_ray(cam.ctypes.data_as(fp), width, height, x, y, C.byref(state), out.ctypes.data_as(fp))
+    return out, state.value
 
 
 def trace_replay(scene: Scene, rays: np.ndarray):

@@ -680,18 +680,39 @@ uint32_t or_path_seed(uint64_t frameSeed, int frameIndex, uint32_t pixelIndex, i
 }
 
 /* One path sample of pixel (x,y): renderKernel body, GpuKernelSources.java:1217-1282. */
-static v3 renderPathSample(const OrScene *s, const float *cam, int width, int height, int x, int y,
-		int maxDepth, int rrStart, int frameIndex, uint64_t frameSeed, int sampleOffset, uint64_t *st) {
-	uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
-	uint32_t rng = or_path_seed(frameSeed, frameIndex, pixel, sampleOffset);
-	float jx = rand01(&rng) - 0.5f;
-	float jy = rand01(&rng) - 0.5f;
+/* Camera ray of renderKernel, GpuKernelSources.java:1256-1264: two jitter
+ * draws from the path's RNG state, the pixel-centre offset of 0.5, the
+ * aspect-scaled u and the downward v, and the unnormalized direction
+ * fwd + right*u + up*v.  ray6 = origin.xyz, direction.xyz. */
+void or_camera_ray(const float *cam, int width, int height, int x, int y, uint32_t *rng, float *ray6) {
+	float jx = rand01(rng) - 0.5f;
+	float jy = rand01(rng) - 0.5f;
 	float aspect = (float) width / (float) height;
 	float u = ((((float) x + 0.5f) + jx) * (2.0f / (float) width) - 1.0f) * aspect;
 	float v = 1.0f - ((((float) y + 0.5f) + jy) * (2.0f / (float) height));
-	v3 pos = mk(cam[0], cam[1], cam[2]), fwd = mk(cam[3], cam[4], cam[5]);
+	v3 fwd = mk(cam[3], cam[4], cam[5]);
 	v3 right = mk(cam[6], cam[7], cam[8]), up = mk(cam[9], cam[10], cam[11]);
 	v3 dir = vadd(vadd(fwd, vmuls(right, u)), vmuls(up, v));
+	ray6[0] = cam[0]; ray6[1] = cam[1]; ray6[2] = cam[2];
+	ray6[3] = dir.x; ray6[4] = dir.y; ray6[5] = dir.z;
+}
+
+/* The camera ray of path (pixel, sample): seed, then or_camera_ray. */
+static void primaryRay(const float *cam, int width, int height, int x, int y, int frameIndex, uint64_t frameSeed,
+		int sampleOffset, uint32_t *rng, v3 *o, v3 *d) {
+	uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
+	float r[6];
+	*rng = or_path_seed(frameSeed, frameIndex, pixel, sampleOffset);
+	or_camera_ray(cam, width, height, x, y, rng, r);
+	*o = mk(r[0], r[1], r[2]);
+	*d = mk(r[3], r[4], r[5]);
+}
+
+static v3 renderPathSample(const OrScene *s, const float *cam, int width, int height, int x, int y,
+		int maxDepth, int rrStart, int frameIndex, uint64_t frameSeed, int sampleOffset, uint64_t *st) {
+	uint32_t rng;
+	v3 pos, dir;
+	primaryRay(cam, width, height, x, y, frameIndex, frameSeed, sampleOffset, &rng, &pos, &dir);
 	return tracePath(s, pos, dir, maxDepth, rrStart, &rng, st);
 }
 
@@ -851,21 +872,15 @@ static v3 adjointPath(const OrScene *s, v3 o, v3 d, int maxDepth, int rrStart, u
 int or_adjoint(const OrScene *s, const float *cam, int width, int height, int maxDepth, int rrStart, int frameIndex,
 		uint64_t frameSeed, int spp, const float *gradImage, double *gAlb, double *gEmit) {
 	if (!s || !cam || !gradImage || !gAlb || !gEmit || spp < 1 || maxDepth > 32) return -1;
-	v3 pos = mk(cam[0], cam[1], cam[2]), fwd = mk(cam[3], cam[4], cam[5]);
-	v3 right = mk(cam[6], cam[7], cam[8]), up = mk(cam[9], cam[10], cam[11]);
 	for (int y = 0; y < height; y++)
 		for (int x = 0; x < width; x++) {
 			uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
 			double delta[3];
 			for (int c = 0; c < 3; c++) delta[c] = (double) gradImage[3 * pixel + c] / (double) spp;
 			for (int so = 0; so < spp; so++) {
-				uint32_t rng = or_path_seed(frameSeed, frameIndex, pixel, so);
-				float jx = rand01(&rng) - 0.5f;
-				float jy = rand01(&rng) - 0.5f;
-				float aspect = (float) width / (float) height;
-				float u = ((((float) x + 0.5f) + jx) * (2.0f / (float) width) - 1.0f) * aspect;
-				float v = 1.0f - ((((float) y + 0.5f) + jy) * (2.0f / (float) height));
-				v3 dir = vadd(vadd(fwd, vmuls(right, u)), vmuls(up, v));
+				uint32_t rng;
+				v3 pos, dir;
+				primaryRay(cam, width, height, x, y, frameIndex, frameSeed, so, &rng, &pos, &dir);
 				adjointPath(s, pos, dir, maxDepth, rrStart, &rng, delta, gAlb, gEmit);
 			}
 		}
@@ -882,20 +897,14 @@ int or_adjoint(const OrScene *s, const float *cam, int width, int height, int ma
 int or_render_fixed_paths64(const OrScene *s, const float *cam, int width, int height, int maxDepth, int frameIndex,
 		uint64_t frameSeed, int spp, const double *mat64, double *out) {
 	if (!s || !cam || !mat64 || !out || spp < 1) return -1;
-	v3 pos = mk(cam[0], cam[1], cam[2]), fwd = mk(cam[3], cam[4], cam[5]);
-	v3 right = mk(cam[6], cam[7], cam[8]), up = mk(cam[9], cam[10], cam[11]);
 	for (int y = 0; y < height; y++)
 		for (int x = 0; x < width; x++) {
 			uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
 			double acc[3] = {0.0, 0.0, 0.0};
 			for (int so = 0; so < spp; so++) {
-				uint32_t rng = or_path_seed(frameSeed, frameIndex, pixel, so);
-				float jx = rand01(&rng) - 0.5f;
-				float jy = rand01(&rng) - 0.5f;
-				float aspect = (float) width / (float) height;
-				float u = ((((float) x + 0.5f) + jx) * (2.0f / (float) width) - 1.0f) * aspect;
-				float v = 1.0f - ((((float) y + 0.5f) + jy) * (2.0f / (float) height));
-				v3 o = pos, d = vadd(vadd(fwd, vmuls(right, u)), vmuls(up, v));
+				uint32_t rng;
+				v3 o, d;
+				primaryRay(cam, width, height, x, y, frameIndex, frameSeed, so, &rng, &o, &d);
 				double T[3] = {1.0, 1.0, 1.0};
 				for (int depth = 0; depth < maxDepth; ++depth) {
 					Hit h;

@@ -81,6 +81,112 @@ def test_path_seed_distinct_and_mixing():
                 O.path_seed(1, 2, 3, 5), O.path_seed(2, 2, 3, 4)}) == 6
 
 
+GOLDEN = 0x9E3779B9
+
+
+def _rand01_ref(state: int) -> tuple[float, int]:
+    """rand01 (GpuKernelSources.java:176-179) from the pinned hash: state = h(state + 0x9e3779b9)."""
+    state = O.hash_u32((state + 0x9E3779B9) & M32)
+    return (state >> 8) * 2.0 ** -24, state
+
+
+def _seed_base(frame_seed: int, frame: int, pixel: int, sample: int) -> int:
+    """Peels the sample layers off a path seed: h^-1(seed) ^ h(s) ^ h(s * 0x9E3779B9 mod 2^32)."""
+    inner = _inv_hash(O.path_seed(frame_seed, frame, pixel, sample))
+    return inner ^ O.hash_u32(sample & M32) ^ O.hash_u32((sample * GOLDEN) & M32)
+
+
+def test_path_seed_sample_layer_is_invariant():
+    """seed = h(base ^ h(s) ^ h(s * 0x9E3779B9)) (GpuKernelSources.java:1250-1253): peeling the final hash and
+    the two sample terms leaves one base per (frame seed, frame, pixel), whatever the sample.  A dropped final
+    hash, a missing or misplaced golden-ratio product or a sample term hashed twice changes the base per s."""
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        fs = int(rng.integers(0, 1 << 63)) | (int(rng.integers(0, 2)) << 63)
+        frame, pixel = int(rng.integers(0, 1 << 20)), int(rng.integers(0, 1 << 31))
+        bases = {_seed_base(fs, frame, pixel, s) for s in list(range(70)) + [1 << 20, (1 << 31) - 1]}
+        assert len(bases) == 1
+    # all-zero inputs: every hash of 0 is 0 (triple32 fixes 0), so the seed is 0
+    assert O.path_seed(0, 0, 0, 0) == 0 and _seed_base(0, 0, 0, 0) == 0
+
+
+def test_path_seed_base_layer_xor_structure():
+    """base = h(h(lo) ^ h(hi) ^ h(frame) ^ h(pixel)) (GpuKernelSources.java:1231-1249): h^-1(base) changes by
+    exactly h(a) ^ h(b) when one input word changes from a to b, for each of the four words, and is 0 for all-zero
+    inputs, which fixes h^-1(base) = h(lo) ^ h(hi) ^ h(frame) ^ h(pixel) everywhere."""
+    H = O.hash_u32
+    rng = np.random.default_rng(12)
+
+    def core(fs, frame, pixel):
+        return _inv_hash(_seed_base(fs, frame, pixel, int(rng.integers(0, 64))))
+
+    assert core(0, 0, 0) == 0
+    for _ in range(60):
+        lo1, lo2, hi1, hi2 = (int(v) for v in rng.integers(0, 1 << 32, 4))
+        f1, f2 = (int(v) for v in rng.integers(0, 1 << 31, 2))
+        p1, p2 = (int(v) for v in rng.integers(0, 1 << 32, 2))
+        fs = hi1 << 32 | lo1
+        assert core(fs, f1, p1) ^ core(fs, f1, p2) == H(p1) ^ H(p2)                 # pixel word
+        assert core(fs, f1, p1) ^ core(fs, f2, p1) == H(f1) ^ H(f2)                 # frame index
+        assert core(hi1 << 32 | lo1, f1, p1) ^ core(hi1 << 32 | lo2, f1, p1) == H(lo1) ^ H(lo2)   # low half
+        assert core(hi1 << 32 | lo1, f1, p1) ^ core(hi2 << 32 | lo1, f1, p1) == H(hi1) ^ H(hi2)   # high half
+    # a negative frame index enters as its 32-bit two's complement
+    assert core(5, -3, 7) ^ core(5, 0, 7) == H((-3) & M32)
+
+
+def _cam(pos=(0.25, -0.5, 2.0), fwd=(0.0, 0.0, 1.0), right=(1.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)):
+    return np.array([*pos, *fwd, *right, *up], np.float32)
+
+
+@pytest.mark.parametrize("w,h", [(1920, 1080), (256, 256), (37, 91)])
+def test_camera_ray_lies_in_its_pixel_footprint(w, h):
+    """renderKernel's camera ray (GpuKernelSources.java:1256-1264) with fwd = z, right = x, up = y has direction
+    (u, v, 1): u / aspect lies in the pixel's footprint [x, x + 1) * 2 / W - 1 and v in 1 - [y, y + 1) * 2 / H,
+    at the offset of the first and second jitter draws of the path's RNG, and the state advances by exactly those
+    two draws.  A missing +0.5, a flipped v, a wrong aspect or a swapped draw fails."""
+    rng = np.random.default_rng(w * 7 + h)
+    aspect = np.float32(w) / np.float32(h)
+    fracx, fracy = [], []
+    for _ in range(1500):
+        x, y, st = int(rng.integers(0, w)), int(rng.integers(0, h)), int(rng.integers(0, 1 << 32))
+        ray, after = O.camera_ray(_cam(), w, h, x, y, st)
+        assert ray[0:3].tolist() == [0.25, -0.5, 2.0] and ray[5] == 1.0
+        jx, s1 = _rand01_ref(st)
+        jy, s2 = _rand01_ref(s1)
+        assert after == s2
+        px = (float(ray[3]) / float(aspect) + 1.0) * w / 2.0
+        py = (1.0 - float(ray[4])) * h / 2.0
+        slack = 4e-6 * max(w, h)
+        assert x - slack <= px <= x + 1 + slack and y - slack <= py <= y + 1 + slack, (x, y, px, py)
+        assert abs((px - x) - jx) <= slack and abs((py - y) - jy) <= slack
+        fracx.append(px - x)
+        fracy.append(py - y)
+    for f in (np.array(fracx), np.array(fracy)):     # jitter uniform over the footprint
+        assert abs(f.mean() - 0.5) < 0.03 and abs(f.var() - 1.0 / 12.0) < 0.01
+        hist, _ = np.histogram(f, bins=10, range=(0, 1))
+        assert hist.min() > 100
+
+
+def test_camera_ray_is_fwd_plus_right_u_plus_up_v():
+    """For a rotated, scaled camera basis the direction is fwd + right u + up v with the (u, v) of the axis
+    camera for the same pixel and state: right carries u, up carries v, fwd enters once and the origin is the
+    camera position."""
+    rng = np.random.default_rng(5)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    fwd, right, up = q[:, 0] * 1.7, q[:, 1] * 0.6, q[:, 2] * 0.9
+    cam = _cam(pos=(1.5, 2.5, -3.0), fwd=fwd, right=right, up=up)
+    for _ in range(300):
+        x, y, st = int(rng.integers(0, 640)), int(rng.integers(0, 480)), int(rng.integers(0, 1 << 32))
+        axis, _ = O.camera_ray(_cam(), 640, 480, x, y, st)
+        ray, _ = O.camera_ray(cam, 640, 480, x, y, st)
+        np.testing.assert_array_equal(ray[0:3], cam[0:3])
+        d = ray[3:6].astype(np.float64) - cam[3:6].astype(np.float64)
+        r64, u64 = cam[6:9].astype(np.float64), cam[9:12].astype(np.float64)
+        u = d @ r64 / (r64 @ r64)
+        v = d @ u64 / (u64 @ u64)
+        assert abs(u - float(axis[3])) < 2e-6 and abs(v - float(axis[4])) < 2e-6
+
+
 def test_frame_seed_is_splitmix64():
     # SplitMix64 of 0 is the published first output of the generator seeded with 0
     assert mrad.mix64(0) == 0xE220A8397B1DCDAF
