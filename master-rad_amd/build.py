@@ -43,11 +43,22 @@ def run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
-def code_object_sha256() -> str:
+def run_all(cmds: list[list[str]]) -> None:
+    """Runs independent compiles side by side (one process each)."""
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(run, cmds))
+
+
+def code_object_sha256(host_cmds: list[list[str]] = ()) -> str:
+    """SHA-256 over the gfx950 code objects of every HIP source; `host_cmds`
+    are compiled alongside."""
     h = hashlib.sha256()
-    for src in HIP_SOURCES:
-        out = BUILD / (Path(src).stem + f"-{ARCH}.hsaco")
-        run([HIPCC, *DEVICE_FLAGS, "--cuda-device-only", "-o", str(out), str(CSRC / src)])
+    outs = [BUILD / (Path(src).stem + f"-{ARCH}.hsaco") for src in HIP_SOURCES]
+    run_all([*host_cmds, *([HIPCC, *DEVICE_FLAGS, "--cuda-device-only", "-o", str(out), str(CSRC / src)]
+                           for src, out in zip(HIP_SOURCES, outs))])
+    for out in outs:
         h.update(out.read_bytes())
     return h.hexdigest()
 
@@ -57,18 +68,20 @@ def build(verbose: bool = True, out: Path | None = None, defines: tuple[str, ...
     `defines` are extra -D tuning macros of the kernels."""
     BUILD.mkdir(exist_ok=True)
     lib = out or LIB
-    objs = []
+    objs, host_cmds = [], []
     for src in CPP_SOURCES:
         obj = BUILD / (Path(src).stem + ".o")
-        run(["g++", *HOST_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)])
+        host_cmds.append(["g++", *HOST_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
-    sha = code_object_sha256()
+    sha = code_object_sha256(host_cmds)
+    cmds = []
     for src in HIP_SOURCES:
         obj = BUILD / (Path(src).stem + ".o")
-        run([HIPCC, *DEVICE_FLAGS, *(f"-D{d}" for d in defines), "-fPIC", f'-DMRAD_CODE_OBJECT_SHA256="{sha}"',
-             "-I", str(ROOT / "include"),
-             "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([HIPCC, *DEVICE_FLAGS, *(f"-D{d}" for d in defines), "-fPIC",
+                     f'-DMRAD_CODE_OBJECT_SHA256="{sha}"', "-I", str(ROOT / "include"),
+                     "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
+    run_all(cmds)
     tmp = lib.with_suffix(".so.tmp")
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, lib)

@@ -190,6 +190,28 @@ struct HostArena {
 			if (s > kRefIndexMask) return set_error(MRAD_ERR_INVALID_ARGUMENT, "arena exceeds 2^30 records");
 		const int mb = (int) matBase;
 		auto mat = [mb](int m) { return m < 0 ? m : m + mb; };
+		// Material indices of the wire records (a float holding an integer):
+		// -1 (no material, GpuScene.materialIndexOf) or an index into this
+		// scene's table.  Anything else, non-finite values included, would
+		// make every kernel read past the table (or another scene's), so the
+		// whole scene is rejected before any value is converted.
+		const int mc = a->material_count;
+		auto matOk = [mc](float v) { return v == -1.0f || (v >= 0.0f && (double) v < (double) mc && v == std::floor(v)); };
+		auto badMat = [](const char *what, int i) {
+			return set_error(MRAD_ERR_INVALID_ARGUMENT,
+			                 std::string(what) + " " + std::to_string(i) + ": material index out of range");
+		};
+		for (int i = 0; i < a->sphere_count; i++)
+			if (!matOk(a->spheres[5 * (size_t) i + 4])) return badMat("sphere", i);
+		for (int i = 0; i < a->box_count; i++)
+			if (!matOk(a->boxes[7 * (size_t) i + 6])) return badMat("box", i);
+		for (int i = 0; i < a->affine_sphere_count; i++)
+			if (!matOk(a->affine_spheres[22 * (size_t) i + 21])) return badMat("affine sphere", i);
+		for (int i = 0; i < a->affine_box_count; i++)
+			if (!matOk(a->affine_boxes[22 * (size_t) i + 21])) return badMat("affine box", i);
+		for (int i = 0; i < a->plane_count; i++)
+			for (int k = 23; k < 26; k++)
+				if (!matOk(a->planes[26 * (size_t) i + k])) return badMat("plane", i);
 		for (int i = 0; i < a->primitive_ref_count; i++) {
 			const uint32_t ref = (uint32_t) a->primitive_refs[i];
 			const uint32_t type = ref & kRefTypeMask;

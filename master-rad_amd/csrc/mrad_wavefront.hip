@@ -942,13 +942,22 @@ int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
 	MRAD_HIP(hipSetDevice(b->cfg.device));
 	dropGraph(b);
 	const bool grow = (size_t) c.queue_capacity * 16 > b->pool[0][0].cap;
-	b->cfg = c;
 	if (grow && !b->dirty) {
-		for (int q = 0; q < 2; q++)
-			for (int k = 0; k < 4; k++) MRAD_TRY(ensure(b->pool[q][k], (size_t) c.queue_capacity * 16));
-		MRAD_TRY(ensure(b->hitA, (size_t) c.queue_capacity * 16));
-		MRAD_TRY(ensure(b->hitB, (size_t) c.queue_capacity * 16));
+		// The pools grow before the configuration changes.  ensure() frees a
+		// buffer before allocating its replacement, so a failed grow leaves
+		// some buffers released: the old configuration stays and the batch is
+		// marked for a full prepare(), which reallocates them at its size.
+		int rc = MRAD_OK;
+		for (int q = 0; q < 2 && rc == MRAD_OK; q++)
+			for (int k = 0; k < 4 && rc == MRAD_OK; k++) rc = ensure(b->pool[q][k], (size_t) c.queue_capacity * 16);
+		if (rc == MRAD_OK) rc = ensure(b->hitA, (size_t) c.queue_capacity * 16);
+		if (rc == MRAD_OK) rc = ensure(b->hitB, (size_t) c.queue_capacity * 16);
+		if (rc != MRAD_OK) {
+			b->dirty = true;
+			return rc;
+		}
 	}
+	b->cfg = c;
 	return MRAD_OK;
 }
 

@@ -39,7 +39,7 @@ from pathlib import Path
 from . import _native as N
 from .recorder import BenchmarkRecorder
 from .scene import BvhBuildConfig
-from .tracer import HeadlessGpuRenderer
+from .tracer import GpuRayTracer, HeadlessGpuRenderer
 
 BENCHMARK_FLUSH_EVERY = 20                   # BenchmarkRunner.java:13
 GPU_VERSIONS = ("GPU_LINEAR", "GPU_BVH", "GPU_WEIGHTED_BVH")
@@ -198,11 +198,14 @@ def base_configuration(phase: str, process_id: int, width: int, height: int, max
 
 
 def gpu_configuration(width: int, height: int, samples_per_frame: int) -> str:
-    """The launch provenance appended for GPU versions (BenchmarkRunner.java:48-59):
-    one launch covers the whole image and every sample of the frame."""
+    """The launch provenance appended for GPU versions (BenchmarkRunner.java:48-59).  As in the reference
+    (GpuLaunchProvenance.SAMPLES_PER_PHYSICAL_KERNEL = 1, physicalRenderLaunches = tiles x samples), each
+    physical launch renders one sample; one launch covers the whole image (one tile), so a logical frame is
+    samples_per_frame launches.  run() builds its renderers with samples_per_launch = 1 to match."""
+    launches = SAMPLES_PER_PHYSICAL_KERNEL * samples_per_frame
     return (f",logicalSamplesPerFrame:{samples_per_frame},renderPixelsPerLaunch:{width * height},"
             f"replayRaysPerLaunch:{width * height},samplesPerPhysicalKernel:{SAMPLES_PER_PHYSICAL_KERNEL},"
-            f"physicalKernelLaunchesPerLogicalFrame:1,kernelNanosSemantics:{KERNEL_NANOS_SEMANTICS}")
+            f"physicalKernelLaunchesPerLogicalFrame:{launches},kernelNanosSemantics:{KERNEL_NANOS_SEMANTICS}")
 
 
 def benchmark_file(directory: str, scene: str, version: str, now: _dt.datetime | None = None) -> Path:
@@ -239,8 +242,15 @@ def host_cpu() -> str:
 
 
 def host_gpu() -> str:
-    import torch
-    return torch.cuda.get_device_name(0) if torch.cuda.is_available() else "unknown GPU"
+    """Name of HIP device 0 from the library's own mrad_device_info; "unknown GPU" without one."""
+    try:
+        with GpuRayTracer(1, 1, 1, True) as t:
+            if t.is_available():
+                name, arch = t.device_info()
+                return name or arch or "unknown GPU"
+    except N.MradError:
+        pass
+    return "unknown GPU"
 
 
 class Plan:
@@ -302,7 +312,8 @@ def run(plan: Plan, out=None, err=None) -> int:
                     renderer = HeadlessGpuRenderer(scene, camera, plan.width, plan.height, plan.max_depth,
                                                    plan.seed, plan.samples_per_frame, version != "GPU_LINEAR",
                                                    collect_metrics=plan.collect_metrics,
-                                                   bvh_stack_size=plan.stack_size)
+                                                   bvh_stack_size=plan.stack_size,
+                                                   samples_per_launch=SAMPLES_PER_PHYSICAL_KERNEL)
                     try:
                         if plan.leaf_batch:
                             renderer.tracer.set_leaf_batch(plan.leaf_batch)

@@ -55,3 +55,38 @@ def assert_replay_equal(gpu, ref, check_counters=True):
     np.testing.assert_array_equal(gpu.hit_normals.view(np.uint32), normals.view(np.uint32))
     if check_counters and gpu.per_ray_counters is not None:
         np.testing.assert_array_equal(gpu.per_ray_counters, per)
+
+
+def corrupt_view(scene, **arrays):
+    """A copy of the scene's wire view (mrad_scene_arrays) with some arrays replaced, e.g.
+    corrupt_view(sc, primitive_refs=refs, spheres=sph).  The replacement numpy arrays are returned with the
+    view so that they outlive it."""
+    import ctypes as C
+
+    from mrad import _native as N
+    v = N.SceneArrays()
+    C.pointer(v)[0] = scene.arrays()
+    keep = []
+    for name, arr in arrays.items():
+        is_int = name in ("primitive_refs", "node_data", "root_indices")
+        keep.append(np.ascontiguousarray(arr, np.int32 if is_int else np.float32))
+        setattr(v, name, keep[-1].ctypes.data_as(N.ip if is_int else N.fp))
+    v.revision = v.revision + 1000
+    return v, keep
+
+
+def bad_material_variants(scene):
+    """Wire arrays of `scene` with one material index out of range (past the table, below -1, non-integral
+    or NaN) in each primitive class the scene has; each item is (label, {array name: array})."""
+    out = []
+    mc = scene.counts()["materials"]
+    for name, data, col in (("spheres", scene.sphere_data(), 4), ("boxes", scene.box_data(), 6),
+                            ("affine_spheres", scene.affine_sphere_data(), 21),
+                            ("affine_boxes", scene.affine_box_data(), 21), ("planes", scene.plane_data(), 24)):
+        if not len(data):
+            continue
+        for bad in (float(mc), -2.0, 0.5, float("nan")):
+            arr = np.array(data, np.float32, copy=True)
+            arr[len(arr) // 2, col] = bad
+            out.append((f"{name}={bad}", {name: arr}))
+    return out

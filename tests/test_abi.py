@@ -11,6 +11,10 @@ import pytest
 
 import mrad
 from mrad import _native as N
+from mrad import configs
+
+from helpers import bad_material_variants, corrupt_view
+from scenes import gi_test_scene, small_field
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -118,3 +122,22 @@ def test_calibration_entry_points_check_arguments_first():
     ms = C.c_float()
     assert N.lib.mrad_cost_time(None, 1, 0, 0, 1, 1, 1, 64, C.byref(ms)) == N.MRAD_ERR_INVALID_ARGUMENT
     N.lib.mrad_cost_destroy(None)
+
+
+@pytest.mark.parametrize("make", [configs.room_scene, gi_test_scene, lambda: small_field(400, 2)])
+def test_scene_with_bad_material_index_is_rejected_on_the_host(make):
+    """A material index that is not -1 (no material, GpuScene.materialIndexOf) or inside the scene's table
+    (past it, below -1, non-integral, NaN) is an invalid argument, found before any value reaches a kernel;
+    mrad_batch_add_scene validates on the host, so no device is needed here."""
+    sc = make()
+    variants = bad_material_variants(sc)
+    assert len(variants) >= 8
+    cam = configs.field_camera()
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=4096)) as b:
+        for label, arrays in variants:
+            v, keep = corrupt_view(sc, **arrays)
+            view = mrad.batch.make_view(cam, 8, 8, 1, 4, 0, 1)
+            rc = N.lib.mrad_batch_add_scene(b._h, C.byref(v), C.byref(view), None)
+            assert rc == N.MRAD_ERR_INVALID_ARGUMENT, label
+            assert b"material index" in N.lib.mrad_last_error(), label
+        assert b.add_scene(sc, cam, 8, 8, 1, 4) == 0    # the valid scene is still accepted, as the first

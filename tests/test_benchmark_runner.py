@@ -32,7 +32,7 @@ def test_plan_from_properties():
     assert text.startswith("phase:timing,processId:2,width:64,height:32,maxDepth:12,bvhLeafSize:8,"
                            "warmupFrames:1,measuredFrames:3,seed:0x5eed,python:")
     assert text.endswith(",logicalSamplesPerFrame:4,renderPixelsPerLaunch:2048,replayRaysPerLaunch:2048,"
-                         "samplesPerPhysicalKernel:1,physicalKernelLaunchesPerLogicalFrame:1,"
+                         "samplesPerPhysicalKernel:1,physicalKernelLaunchesPerLogicalFrame:4,"
                          "kernelNanosSemantics:" + B.KERNEL_NANOS_SEMANTICS)
     assert plan.configs["GPU_BVH"] == mrad.BvhBuildConfig.uniform(8)
     assert plan.configs["GPU_WEIGHTED_BVH"] == mrad.BvhBuildConfig.weighted(8, 1.0)

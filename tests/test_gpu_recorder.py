@@ -76,6 +76,7 @@ def test_benchmark_runner_writes_one_file_per_pair(require_gpu, tmp_path, capsys
     for f, mode in zip(files, ("mode=LINEAR", "mode=WEIGHTED_SAH")):
         lines = f.read_text().splitlines()
         assert f"accelerationConfig={mode}" in lines[0] and "config=phase:counters," in lines[0]
+        assert ",samplesPerPhysicalKernel:1,physicalKernelLaunchesPerLogicalFrame:2," in lines[0]
         assert lines[1] == ",".join(R.COLUMNS) and len(lines) == 4
         rows = [dict(zip(R.COLUMNS, line.split(","))) for line in lines[2:]]
         assert all(r["count"] == "2" and int(r["cameraSamples"]) == 64 * 32 * 2 for r in rows)

@@ -12,7 +12,8 @@ import mrad
 from mrad import _native as N
 from mrad import configs
 
-from scenes import small_field
+from helpers import bad_material_variants, corrupt_view
+from scenes import gi_test_scene, small_field
 
 pytestmark = pytest.mark.gpu
 
@@ -76,3 +77,19 @@ def test_batch_arena_unchanged_after_rejected_scene(require_gpu):
         b.add_scene(good, cam, 40, 30, 2, 6, 0, seed)
         b.render()
         np.testing.assert_array_equal(b.image(0).view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("make", [configs.room_scene, gi_test_scene])
+def test_tracer_rejects_bad_material_indices(require_gpu, make):
+    """mrad_upload_scene rejects a material index outside -1 / [0, materials) before any launch, and the
+    tracer renders a valid scene afterwards."""
+    sc = make()
+    with mrad.GpuRayTracer(8, 8, 1, True) as t:
+        for label, arrays in bad_material_variants(sc):
+            v, keep = corrupt_view(sc, **arrays)
+            rc = N.lib.mrad_upload_scene(t._h, C.byref(v), C.c_uint64(77))
+            assert rc == N.MRAD_ERR_INVALID_ARGUMENT, label
+            assert b"material index" in N.lib.mrad_last_error()
+        img = np.zeros(8 * 8 * 3, np.float32)
+        t.render_sample(img, sc, configs.room_camera(), 4, 0, 1)
+        assert np.isfinite(img).all()
