@@ -180,6 +180,18 @@ int mrad_copy_frame(mrad_tracer *tracer, float *rgb_out);
 int mrad_trace_replay(mrad_tracer *tracer, const float *rays, int n, int collect_metrics,
                       int32_t *hit_flags, float *hit_distances, int32_t *hit_primitive_orders,
                       int32_t *hit_material_indices, float *hit_normals, uint32_t *per_ray_counters);
+/* No counterpart in the reference.  mrad_trace_replay for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, results and errors, but the
+ * device staging of the rays, results and counters lives in one static device
+ * array per GPU (6.8 MiB, held until the process ends), so the call makes no
+ * hipMalloc or hipFree.  n above MRAD_NOALLOC_MAX_RAYS is an invalid argument.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (two tracer handles used from two threads) race. */
+#define MRAD_NOALLOC_MAX_RAYS 65536
+int mrad_trace_replay_noalloc(mrad_tracer *tracer, const float *rays, int n, int collect_metrics,
+                              int32_t *hit_flags, float *hit_distances, int32_t *hit_primitive_orders,
+                              int32_t *hit_material_indices, float *hit_normals, uint32_t *per_ray_counters);
 /* GpuRayTracer.lastFrameStats() (GpuRayTracer.java:298-307). */
 int mrad_get_frame_stats(mrad_tracer *tracer, mrad_frame_stats *out);
 /* GpuRayTracer.deviceInfo() (GpuRayTracer.java:309-311): name and gfx arch. */

@@ -179,6 +179,125 @@ using namespace mrad;
 
 namespace {
 
+// Device staging of one replay call: rays in; flags, distances, orders,
+// material indices, normals, per-ray counters and the frame counters out.
+struct ReplayBuffers {
+	float *rays, *dist, *normals;
+	int *flags, *order, *mat;
+	uint32_t *perRay;
+	unsigned long long *stats;
+};
+
+int replayChecks(mrad_tracer *t, const float *rays, int n, int32_t *flags, float *dist, int32_t *order, int32_t *mat,
+                 float *normals, uint32_t *) {
+	if (!rays || n <= 0) return set_error(MRAD_ERR_INVALID_ARGUMENT, "Replay rays must contain origin.xyz and direction.xyz.");
+	if (!flags || !dist || !order || !mat || !normals) return set_error(MRAD_ERR_INVALID_ARGUMENT, "replay outputs must not be null");
+	return requireTracer(t);
+}
+
+// Checks that need the handle's lock held.
+int replayState(mrad_tracer *t, uint32_t *perRay) {
+	if (!t->hasScene) return set_error(MRAD_ERR_ILLEGAL_STATE, "no scene is resident: call mrad_upload_scene first");
+	if (perRay && !t->cfg.collect_metrics)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "per-ray counters need a tracer created with collect_metrics");
+	return MRAD_OK;
+}
+
+// One replay launch on staging B: rays copied in, the kernel timed with the
+// handle's events, results copied out, frame statistics updated.  The
+// caller holds the handle's lock with its device current.
+int replayWith(mrad_tracer *t, const ReplayBuffers &B, const float *rays, int n, int collect, int32_t *flags,
+               float *dist, int32_t *order, int32_t *mat, float *normals, uint32_t *perRay) {
+	const int64_t t0 = nowNanos();
+	const size_t N = (size_t) n;
+	const bool metrics = collect && t->cfg.collect_metrics;
+	MRAD_HIP(hipMemcpyAsync(B.rays, rays, N * 24, hipMemcpyHostToDevice, t->stream));
+	MRAD_HIP(hipMemsetAsync(B.stats, 0, MRAD_STAT_COUNT * 8, t->stream));
+	ReplayParams P{};
+	P.S = t->arena.view;
+	P.R = t->range;
+	P.rays = B.rays;
+	P.n = n;
+	P.flags = B.flags;
+	P.dist = B.dist;
+	P.order = B.order;
+	P.mat = B.mat;
+	P.normals = B.normals;
+	P.perRay = perRay ? B.perRay : nullptr;
+	const bool instrumented = metrics || perRay;
+	P.stats = instrumented ? B.stats : nullptr;
+	P.matBase = 0;
+	ReplayFn fn = instrumented ? replayFor<true>(t->cfg.bvh_stack_size) : replayFor<false>(t->cfg.bvh_stack_size);
+	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
+	hipLaunchKernelGGL(fn, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, t->stream, P);
+	MRAD_HIP(hipGetLastError());
+	MRAD_HIP(hipEventRecord(t->evEnd, t->stream));
+	MRAD_HIP(hipEventSynchronize(t->evEnd));
+	float ms = 0.0f;
+	MRAD_HIP(hipEventElapsedTime(&ms, t->evStart, t->evEnd));
+	const int64_t c0 = nowNanos();
+	MRAD_HIP(hipMemcpy(flags, B.flags, N * 4, hipMemcpyDeviceToHost));
+	MRAD_HIP(hipMemcpy(dist, B.dist, N * 4, hipMemcpyDeviceToHost));
+	MRAD_HIP(hipMemcpy(order, B.order, N * 4, hipMemcpyDeviceToHost));
+	MRAD_HIP(hipMemcpy(mat, B.mat, N * 4, hipMemcpyDeviceToHost));
+	MRAD_HIP(hipMemcpy(normals, B.normals, N * 12, hipMemcpyDeviceToHost));
+	if (perRay) MRAD_HIP(hipMemcpy(perRay, B.perRay, N * MRAD_STAT_COUNT * 4, hipMemcpyDeviceToHost));
+	if (instrumented) MRAD_HIP(hipMemcpy(t->last.counters, B.stats, sizeof t->last.counters, hipMemcpyDeviceToHost));
+	else std::memset(t->last.counters, 0, sizeof t->last.counters);
+	t->last.copy_nanos = nowNanos() - c0;
+	t->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
+	t->last.maximum_physical_kernel_nanos = t->last.kernel_nanos;
+	t->last.physical_launches = 1;
+	t->last.total_nanos = nowNanos() - t0;
+	return MRAD_OK;
+}
+
+// Scratch of mrad_trace_replay_noalloc, one copy per GPU for the life of the
+// process: the 14 frame counters, then for up to MRAD_NOALLOC_MAX_RAYS rays
+// the ray staging (24 B), flags, distances, orders, material indices (4 B
+// each), normals (12 B) and per-ray counters (56 B); 6.8 MiB in all.  Every
+// call on a GPU shares that GPU's copy.
+constexpr size_t kNoAllocRays = MRAD_NOALLOC_MAX_RAYS;
+constexpr size_t scratchAlign(size_t v) { return (v + 255) & ~(size_t) 255; }
+constexpr size_t kScratchStats = 0;
+constexpr size_t kScratchRays = scratchAlign(kScratchStats + MRAD_STAT_COUNT * 8);
+constexpr size_t kScratchFlags = scratchAlign(kScratchRays + kNoAllocRays * 24);
+constexpr size_t kScratchDist = scratchAlign(kScratchFlags + kNoAllocRays * 4);
+constexpr size_t kScratchOrder = scratchAlign(kScratchDist + kNoAllocRays * 4);
+constexpr size_t kScratchMat = scratchAlign(kScratchOrder + kNoAllocRays * 4);
+constexpr size_t kScratchNormals = scratchAlign(kScratchMat + kNoAllocRays * 4);
+constexpr size_t kScratchPerRay = scratchAlign(kScratchNormals + kNoAllocRays * 12);
+constexpr size_t kScratchBytes = scratchAlign(kScratchPerRay + kNoAllocRays * MRAD_STAT_COUNT * 4);
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// The device address of `scratch` on each GPU.  In host code the array's
+// name is only a host-side stand-in, so kernels and copies use the address
+// hipGetSymbolAddress returns, taken once per device on the first call there
+// (with that device current).
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMu;
+unsigned char *g_scratchPtr[kMaxScratchDevices] = {};
+
+int scratchFor(int device, unsigned char **out) {
+	if (device < 0 || device >= kMaxScratchDevices)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "no-allocation scratch supports devices 0.." +
+		                                                std::to_string(kMaxScratchDevices - 1));
+	std::lock_guard<std::mutex> g(g_scratchMu);
+	if (!g_scratchPtr[device]) {
+		void *p = nullptr;
+		MRAD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (!p) return set_error(MRAD_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+		g_scratchPtr[device] = static_cast<unsigned char *>(p);
+	}
+	*out = g_scratchPtr[device];
+	return MRAD_OK;
+}
+
 int ensureEvents(mrad_tracer *t, int boundaries) {
 	while ((int) t->boundary.size() < boundaries) {
 		hipEvent_t e;
@@ -393,15 +512,10 @@ int mrad_copy_frame(mrad_tracer *t, float *rgb) {
 int mrad_trace_replay(mrad_tracer *t, const float *rays, int n, int collect, int32_t *flags, float *dist, int32_t *order,
                       int32_t *mat, float *normals, uint32_t *perRay) {
 	clear_error();
-	if (!rays || n <= 0) return set_error(MRAD_ERR_INVALID_ARGUMENT, "Replay rays must contain origin.xyz and direction.xyz.");
-	if (!flags || !dist || !order || !mat || !normals) return set_error(MRAD_ERR_INVALID_ARGUMENT, "replay outputs must not be null");
-	MRAD_TRY(requireTracer(t));
+	MRAD_TRY(replayChecks(t, rays, n, flags, dist, order, mat, normals, perRay));
 	std::lock_guard<std::mutex> g(t->mu);
-	if (!t->hasScene) return set_error(MRAD_ERR_ILLEGAL_STATE, "no scene is resident: call mrad_upload_scene first");
-	if (perRay && !t->cfg.collect_metrics)
-		return set_error(MRAD_ERR_INVALID_ARGUMENT, "per-ray counters need a tracer created with collect_metrics");
+	MRAD_TRY(replayState(t, perRay));
 	MRAD_HIP(hipSetDevice(t->cfg.device));
-	const int64_t t0 = nowNanos();
 	const size_t N = (size_t) n;
 	MRAD_TRY(ensure(t->rays, N * 24));
 	MRAD_TRY(ensure(t->flags, N * 4));
@@ -410,47 +524,44 @@ int mrad_trace_replay(mrad_tracer *t, const float *rays, int n, int collect, int
 	MRAD_TRY(ensure(t->mat, N * 4));
 	MRAD_TRY(ensure(t->normals, N * 12));
 	MRAD_TRY(ensure(t->stats, MRAD_STAT_COUNT * 8));
-	const bool metrics = collect && t->cfg.collect_metrics;
 	if (perRay) MRAD_TRY(ensure(t->perRay, N * MRAD_STAT_COUNT * 4));
-	MRAD_HIP(hipMemcpyAsync(t->rays.p, rays, N * 24, hipMemcpyHostToDevice, t->stream));
-	MRAD_HIP(hipMemsetAsync(t->stats.p, 0, MRAD_STAT_COUNT * 8, t->stream));
-	ReplayParams P{};
-	P.S = t->arena.view;
-	P.R = t->range;
-	P.rays = t->rays.as<const float>();
-	P.n = n;
-	P.flags = t->flags.as<int>();
-	P.dist = t->dist.as<float>();
-	P.order = t->order.as<int>();
-	P.mat = t->mat.as<int>();
-	P.normals = t->normals.as<float>();
-	P.perRay = perRay ? t->perRay.as<uint32_t>() : nullptr;
-	const bool instrumented = metrics || perRay;
-	P.stats = instrumented ? t->stats.as<unsigned long long>() : nullptr;
-	P.matBase = 0;
-	ReplayFn fn = instrumented ? replayFor<true>(t->cfg.bvh_stack_size) : replayFor<false>(t->cfg.bvh_stack_size);
-	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
-	hipLaunchKernelGGL(fn, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, t->stream, P);
-	MRAD_HIP(hipGetLastError());
-	MRAD_HIP(hipEventRecord(t->evEnd, t->stream));
-	MRAD_HIP(hipEventSynchronize(t->evEnd));
-	float ms = 0.0f;
-	MRAD_HIP(hipEventElapsedTime(&ms, t->evStart, t->evEnd));
-	const int64_t c0 = nowNanos();
-	MRAD_HIP(hipMemcpy(flags, t->flags.p, N * 4, hipMemcpyDeviceToHost));
-	MRAD_HIP(hipMemcpy(dist, t->dist.p, N * 4, hipMemcpyDeviceToHost));
-	MRAD_HIP(hipMemcpy(order, t->order.p, N * 4, hipMemcpyDeviceToHost));
-	MRAD_HIP(hipMemcpy(mat, t->mat.p, N * 4, hipMemcpyDeviceToHost));
-	MRAD_HIP(hipMemcpy(normals, t->normals.p, N * 12, hipMemcpyDeviceToHost));
-	if (perRay) MRAD_HIP(hipMemcpy(perRay, t->perRay.p, N * MRAD_STAT_COUNT * 4, hipMemcpyDeviceToHost));
-	if (instrumented) MRAD_HIP(hipMemcpy(t->last.counters, t->stats.p, sizeof t->last.counters, hipMemcpyDeviceToHost));
-	else std::memset(t->last.counters, 0, sizeof t->last.counters);
-	t->last.copy_nanos = nowNanos() - c0;
-	t->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
-	t->last.maximum_physical_kernel_nanos = t->last.kernel_nanos;
-	t->last.physical_launches = 1;
-	t->last.total_nanos = nowNanos() - t0;
-	return MRAD_OK;
+	ReplayBuffers B;
+	B.rays = t->rays.as<float>();
+	B.flags = t->flags.as<int>();
+	B.dist = t->dist.as<float>();
+	B.order = t->order.as<int>();
+	B.mat = t->mat.as<int>();
+	B.normals = t->normals.as<float>();
+	B.perRay = t->perRay.as<uint32_t>();
+	B.stats = t->stats.as<unsigned long long>();
+	return replayWith(t, B, rays, n, collect, flags, dist, order, mat, normals, perRay);
+}
+
+// The replay of mrad_trace_replay with its device staging in the library's
+// static scratch array instead of buffers it allocates.
+int mrad_trace_replay_noalloc(mrad_tracer *t, const float *rays, int n, int collect, int32_t *flags, float *dist,
+                              int32_t *order, int32_t *mat, float *normals, uint32_t *perRay) {
+	clear_error();
+	MRAD_TRY(replayChecks(t, rays, n, flags, dist, order, mat, normals, perRay));
+	if (n > MRAD_NOALLOC_MAX_RAYS)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "mrad_trace_replay_noalloc takes at most " +
+		                                                std::to_string(MRAD_NOALLOC_MAX_RAYS) + " rays per call, got " +
+		                                                std::to_string(n));
+	std::lock_guard<std::mutex> g(t->mu);
+	MRAD_TRY(replayState(t, perRay));
+	MRAD_HIP(hipSetDevice(t->cfg.device));
+	unsigned char *base = nullptr;
+	MRAD_TRY(scratchFor(t->cfg.device, &base));
+	ReplayBuffers B;
+	B.stats = reinterpret_cast<unsigned long long *>(base + kScratchStats);
+	B.rays = reinterpret_cast<float *>(base + kScratchRays);
+	B.flags = reinterpret_cast<int *>(base + kScratchFlags);
+	B.dist = reinterpret_cast<float *>(base + kScratchDist);
+	B.order = reinterpret_cast<int *>(base + kScratchOrder);
+	B.mat = reinterpret_cast<int *>(base + kScratchMat);
+	B.normals = reinterpret_cast<float *>(base + kScratchNormals);
+	B.perRay = reinterpret_cast<uint32_t *>(base + kScratchPerRay);
+	return replayWith(t, B, rays, n, collect, flags, dist, order, mat, normals, perRay);
 }
 
 int mrad_get_frame_stats(mrad_tracer *t, mrad_frame_stats *out) {

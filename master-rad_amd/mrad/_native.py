@@ -95,6 +95,7 @@ SIGNATURES = {
     "mrad_render_sample": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64]),
     "mrad_copy_frame": (C.c_int, [C.c_void_p, fp]),
     "mrad_trace_replay": (C.c_int, [C.c_void_p, fp, C.c_int, C.c_int, ip, fp, ip, ip, fp, up]),
+    "mrad_trace_replay_noalloc": (C.c_int, [C.c_void_p, fp, C.c_int, C.c_int, ip, fp, ip, ip, fp, up]),
     "mrad_get_frame_stats": (C.c_int, [C.c_void_p, C.POINTER(FrameStats)]),
     "mrad_device_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
     "mrad_code_object_sha256": (C.c_int, [C.c_char_p]),
