@@ -247,6 +247,8 @@ typedef struct {
 	int time_kernels;        /* time each kernel with HIP events (no graph; for measurement) */
 	int lds_stack_entries;   /* traversal-stack entries per lane held in LDS: 4 or 10 (0 = 10) */
 	int leaf_regroup;        /* 1 = regroup each wave's leaf tests by primitive class */
+	int result_chunk_paths;  /* paths of whole pixels whose radiance is held for the ordered per-pixel sum, per
+	                            result buffer (two are kept): 0 = 8 pools' worth in [2^24, 2^30]; at most 2^30 */
 } mrad_wavefront_config;
 
 typedef struct {
@@ -261,7 +263,10 @@ typedef struct mrad_batch mrad_batch;
 
 int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out);
 void mrad_batch_destroy(mrad_batch *batch);
-/* Appends a scene (wire arrays copied) with its view; index_out may be NULL. */
+/* Appends a scene (wire arrays copied) with its view; index_out may be NULL.
+ * A view has at most 2^32 pixels and 2^30 samples; the batch's paths are
+ * numbered with 64 bits, so its size is bounded by device memory for the
+ * scenes and images only. */
 int mrad_batch_add_scene(mrad_batch *batch, const mrad_scene_arrays *scene, const mrad_view *view, int *index_out);
 /* Renders every scene of the batch; images stay on the device. */
 int mrad_batch_render(mrad_batch *batch);

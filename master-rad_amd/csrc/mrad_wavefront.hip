@@ -12,10 +12,18 @@
 //             and prefix sum with one atomic per wave, finished paths write
 //             their radiance to a per-path slot.
 // Two bounce steps (pool A to B and back) are captured once as a hipGraph and
-// replayed until every path has finished; a resolve kernel then sums each
-// pixel's samples in sample order.  Every path runs exactly the megakernel's
-// arithmetic and every pixel sums its samples in the megakernel's order, so
-// both forms produce bit-identical images.
+// replayed until every path has finished.  Every path runs exactly the
+// megakernel's arithmetic and every pixel sums its samples in the megakernel's
+// order, so both forms produce bit-identical images.
+//
+// Paths are numbered in global order (scene, pixel, sample) with 64-bit ids,
+// and the batch is cut into chunks of whole pixels.  A finished path writes
+// its radiance to a slot of one of two result buffers (chunk c uses buffer
+// c mod 2); once every path of a chunk has finished, a resolve kernel sums
+// each of its pixels' samples in sample order and the buffer is free for chunk
+// c + 2.  Chunk c + 2 is generated only after chunk c is resolved, while chunk
+// c + 1 keeps the pool full, so result memory is bounded by two chunks (a
+// multiple of the pool), not by the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,20 +43,34 @@ struct WfScene {
 	float cam[12];
 	int width, height, spp, maxDepth, rrStart, frameIndex;
 	uint64_t frameSeed;
-	uint32_t pathBase;      // first global path id of the scene
-	uint32_t pixelBase;     // first global pixel of the scene
+	uint64_t pathBase;      // first global path id of the scene
+	uint64_t pixelBase;     // first global pixel of the scene
+};
+
+// A run of whole pixels and their paths; its radiance goes to result buffer
+// (index mod 2).
+struct WfChunk {
+	uint64_t pathStart, pixelStart;
+	uint32_t pathCount, pixelCount;
 };
 
 // Path pool, structure of arrays of 16-byte records:
 //   od0 = (o.x, o.y, o.z, d.x)   od1 = (d.y, d.z, T.x, T.y)
-//   tl  = (T.z, L.x, L.y, L.z)   meta = (rng, path id, scene | depth << 16, 0)
+//   tl  = (T.z, L.x, L.y, L.z)   meta = (rng, result slot, scene | depth << 16, 0)
+// The result slot is buffer * chunkPaths + the path's index in its chunk.
 struct WfPool {
 	float4 *od0, *od1, *tl;
 	uint4 *meta;
 };
 
-// Device counter block.
-enum { kCount0 = 0, kCount1 = 1, kNextPath = 2, kWorkTrace = 3, kCounterWords = 16 };
+// Device counter block: pool counts, the chunk being generated and how many
+// of its paths are generated, how many chunks may be generated (raised by the
+// host as chunks are resolved), the trace kernel's work counter, and the
+// finished paths of the chunk in each result buffer.
+enum {
+	kCount0 = 0, kCount1 = 1, kGenChunk = 2, kGenOffset = 3, kGenLimit = 4, kWorkTrace = 5, kDone0 = 6, kDone1 = 7,
+	kCounterWords = 16
+};
 constexpr uint32_t kDead = 0xFFFFu;          // depth field of a path that never traces
 constexpr uint32_t kMissKind = 7u;
 
@@ -56,19 +78,20 @@ struct WfParams {
 	DevScene S;
 	const WfScene *scenes;
 	int sceneCount;
+	const WfChunk *chunks;
+	uint32_t chunkCount;
+	uint32_t chunkPaths;    // result slots per buffer
 	WfPool pool[2];
 	float4 *hitA;           // resolved closest hit: position, material index bits
 	float4 *hitB;           //                       normal, 1 if hit else 0
-	float *results;         // 3 floats per path
+	float *results;         // 3 floats per result slot, two buffers of chunkPaths slots
 	float *images;          // 3 floats per pixel, all scenes
 	uint32_t *counters;
-	uint32_t totalPaths;
-	uint32_t totalPixels;
 	uint32_t capacity;
 	unsigned long long *stats;
 };
 
-__device__ __forceinline__ int sceneOfPath(const WfParams &P, uint32_t g) {
+__device__ __forceinline__ int sceneOfPath(const WfParams &P, uint64_t g) {
 	int lo = 0, hi = P.sceneCount - 1;
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
@@ -76,7 +99,7 @@ __device__ __forceinline__ int sceneOfPath(const WfParams &P, uint32_t g) {
 	}
 	return lo;
 }
-__device__ __forceinline__ int sceneOfPixel(const WfParams &P, uint32_t p) {
+__device__ __forceinline__ int sceneOfPixel(const WfParams &P, uint64_t p) {
 	int lo = 0, hi = P.sceneCount - 1;
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
@@ -85,43 +108,52 @@ __device__ __forceinline__ int sceneOfPixel(const WfParams &P, uint32_t p) {
 	return lo;
 }
 
+// Paths the next generate step appends to pool `cur`: up to the pool's free
+// space, from the current chunk only, and none from a chunk the host has not
+// released yet.
 __device__ __forceinline__ uint32_t genCount(const WfParams &P, int cur) {
+	const uint32_t chunk = P.counters[kGenChunk];
+	if (chunk >= P.chunkCount || chunk >= P.counters[kGenLimit]) return 0u;
 	const uint32_t base = P.counters[kCount0 + cur];
-	const uint32_t next = P.counters[kNextPath];
 	const uint32_t free = P.capacity > base ? P.capacity - base : 0u;
-	const uint32_t left = P.totalPaths - next;
+	const uint32_t left = P.chunks[chunk].pathCount - P.counters[kGenOffset];
 	return free < left ? free : left;
 }
 
-// Camera paths [next, next + n) appended to pool `cur` (renderKernel's seed
-// and camera ray, GpuKernelSources.java:1231-1264).
+// Camera paths of the current chunk appended to pool `cur` (renderKernel's
+// seed and camera ray, GpuKernelSources.java:1231-1264).
 __global__ __launch_bounds__(kBlock) void wfGenerate(WfParams P, int cur) {
-	const uint32_t base = P.counters[kCount0 + cur];
-	const uint32_t next = P.counters[kNextPath];
 	const uint32_t n = genCount(P, cur);
+	if (n == 0) return;
+	const uint32_t base = P.counters[kCount0 + cur];
+	const uint32_t chunk = P.counters[kGenChunk];
+	const uint32_t offset = P.counters[kGenOffset];
+	const uint64_t first = P.chunks[chunk].pathStart + offset;
+	const uint32_t slotBase = (chunk & 1u) * P.chunkPaths + offset;
 	const WfPool Q = P.pool[cur];
 	for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-		const uint32_t g = next + j;
+		const uint64_t g = first + j;
 		const int k = sceneOfPath(P, g);
 		const WfScene &V = P.scenes[k];
-		const uint32_t local = g - V.pathBase;
-		const uint32_t pixel = local / (uint32_t) V.spp, s = local % (uint32_t) V.spp;
+		const uint64_t local = g - V.pathBase;
+		const uint32_t pixel = (uint32_t) (local / (uint64_t) V.spp), s = (uint32_t) (local % (uint64_t) V.spp);
 		const int x = (int) (pixel % (uint32_t) V.width), y = (int) (pixel / (uint32_t) V.width);
 		uint32_t rng = path_seed(V.frameSeed, V.frameIndex, pixel, (int) s);
 		f3 o, d;
 		cameraRay(V.cam, V.width, V.height, x, y, rng, o, d);
+		const uint32_t result = slotBase + j;
 		uint32_t depth = 0;
 		if (V.maxDepth == 0) {
 			depth = kDead;
-			P.results[3 * (size_t) g] = 0.0f;
-			P.results[3 * (size_t) g + 1] = 0.0f;
-			P.results[3 * (size_t) g + 2] = 0.0f;
+			P.results[3 * (size_t) result] = 0.0f;
+			P.results[3 * (size_t) result + 1] = 0.0f;
+			P.results[3 * (size_t) result + 2] = 0.0f;
 		}
 		const uint32_t slot = base + j;
 		Q.od0[slot] = make_float4(o.x, o.y, o.z, d.x);
 		Q.od1[slot] = make_float4(d.y, d.z, 1.0f, 1.0f);
 		Q.tl[slot] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-		Q.meta[slot] = make_uint4(rng, g, (uint32_t) k | (depth << 16), 0u);
+		Q.meta[slot] = make_uint4(rng, result, (uint32_t) k | (depth << 16), 0u);
 	}
 }
 
@@ -129,8 +161,30 @@ __global__ void wfAdvanceGenerate(WfParams P, int cur) {
 	if (threadIdx.x != 0 || blockIdx.x != 0) return;
 	const uint32_t n = genCount(P, cur);
 	P.counters[kCount0 + cur] += n;
-	P.counters[kNextPath] += n;
+	if (n > 0) {
+		const uint32_t chunk = P.counters[kGenChunk];
+		const uint32_t offset = P.counters[kGenOffset] + n;
+		const bool done = offset == P.chunks[chunk].pathCount;
+		P.counters[kGenOffset] = done ? 0u : offset;
+		P.counters[kGenChunk] = done ? chunk + 1u : chunk;
+	}
 	P.counters[kWorkTrace] = 0;
+}
+
+// Counter block at the start of a render: zero, with chunks 0 and 1 released.
+__global__ void wfInitCounters(WfParams P) {
+	if (blockIdx.x != 0 || threadIdx.x >= kCounterWords) return;
+	P.counters[threadIdx.x] = threadIdx.x == kGenLimit ? 2u : 0u;
+}
+
+// Chunks [first, last) have been resolved: their buffers' finished counts
+// start again from zero and chunks up to last + 1 may be generated.  The
+// counters are updated by this kernel rather than by host copies, so every
+// write to them is ordered with the kernels' atomics on the device.
+__global__ void wfRelease(WfParams P, uint32_t first, uint32_t last) {
+	if (threadIdx.x != 0 || blockIdx.x != 0) return;
+	for (uint32_t c = first; c < last && c < first + 2; c++) P.counters[kDone0 + (c & 1u)] = 0u;
+	P.counters[kGenLimit] = last + 2u;
 }
 
 __global__ void wfAdvanceShade(WfParams P, int cur) {
@@ -200,13 +254,18 @@ void wfTrace(WfParams P, int cur) {
 
 // Shading of one pool entry; true when the path continues (its new state is
 // left in od0, od1, tl, meta), false when it finished (its radiance is written
-// to its result slot) or the entry is empty.
+// to its result slot) or the entry is empty.  A path that finishes here, or
+// that never traced (depth 0), adds one to finished[its result buffer].
 __device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, uint32_t i, uint32_t count, float4 &od0,
-                                           float4 &od1, float4 &tl, uint4 &meta) {
+                                           float4 &od1, float4 &tl, uint4 &meta, uint32_t (&finished)[2]) {
 	if (i >= count) return false;
 	meta = Q.meta[i];
 	const uint32_t depth = meta.z >> 16;
-	if (depth == kDead) return false;
+	const uint32_t buffer = meta.y >= P.chunkPaths ? 1u : 0u;
+	if (depth == kDead) {
+		finished[buffer]++;
+		return false;
+	}
 	od0 = Q.od0[i];
 	od1 = Q.od1[i];
 	tl = Q.tl[i];
@@ -233,6 +292,7 @@ __device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, u
 		r[0] = L.x;
 		r[1] = L.y;
 		r[2] = L.z;
+		finished[buffer]++;
 	}
 	return alive;
 }
@@ -263,10 +323,13 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 #endif
 	__shared__ uint32_t part[kBuckets * kShadePer * kWaves];
 	__shared__ uint32_t tileBase;
+	__shared__ uint32_t doneLds[2];
 	const uint32_t count = P.counters[kCount0 + cur];
 	const WfPool Q = P.pool[cur], D = P.pool[cur ^ 1];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const unsigned long long below = (1ull << lane) - 1ull;
+	uint32_t finished[2] = {0u, 0u};
+	if (threadIdx.x < 2) doneLds[threadIdx.x] = 0u;
 	for (uint32_t tile = blockIdx.x * kShadeTile; tile < count; tile += gridDim.x * kShadeTile) {
 		float4 od0[kShadePer], od1[kShadePer], tl[kShadePer];
 		uint4 meta[kShadePer];
@@ -275,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 		unsigned long long mask[kShadePer];
 #pragma unroll
 		for (int k = 0; k < kShadePer; k++) {
-			alive[k] = shadeEntry(P, Q, tile + k * kBlock + threadIdx.x, count, od0[k], od1[k], tl[k], meta[k]);
+			alive[k] = shadeEntry(P, Q, tile + k * kBlock + threadIdx.x, count, od0[k], od1[k], tl[k], meta[k],
+			                      finished);
 			bucket[k] = (kBuckets == 1 || !alive[k]) ? 0
 			          : (od0[k].w < 0.0f ? 1 : 0) | (od1[k].x < 0.0f ? 2 : 0) | (od1[k].y < 0.0f ? 4 : 0);
 			for (int q = 0; q < kBuckets; q++) {
@@ -308,13 +372,27 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 		}
 		__syncthreads();
 	}
+	// finished paths per result buffer: one LDS sum per block, one atomic per
+	// block and buffer
+	__syncthreads();
+	for (int b = 0; b < 2; b++) {
+		uint32_t v = finished[b];
+		for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+		if (lane == 0 && v) atomicAdd(&doneLds[b], v);
+	}
+	__syncthreads();
+	if (threadIdx.x < 2 && doneLds[threadIdx.x]) atomicAdd(P.counters + kDone0 + threadIdx.x, doneLds[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(kBlock) void wfResolve(WfParams P) {
-	for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < P.totalPixels; p += gridDim.x * kBlock) {
+// Sums the samples of every pixel of chunk `c` in sample order.
+__global__ __launch_bounds__(kBlock) void wfResolve(WfParams P, uint32_t c) {
+	const WfChunk C = P.chunks[c];
+	const size_t slotBase = (size_t) (c & 1u) * P.chunkPaths;
+	for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < C.pixelCount; j += gridDim.x * kBlock) {
+		const uint64_t p = C.pixelStart + j;
 		const WfScene &V = P.scenes[sceneOfPixel(P, p)];
-		const uint32_t pixel = p - V.pixelBase;
-		const size_t first = (size_t) V.pathBase + (size_t) pixel * V.spp;
+		const uint64_t pixel = p - V.pixelBase;
+		const size_t first = slotBase + (size_t) (V.pathBase + pixel * (uint64_t) V.spp - C.pathStart);
 		const float *r = P.results + 3 * first;
 		f3 acc = mk(0.0f, 0.0f, 0.0f);
 		if ((V.spp & 3) == 0 && (first & 3) == 0) {
@@ -632,11 +710,14 @@ struct mrad_batch {
 	std::vector<WfScene> scenes;
 	bool dirty = true;
 	DevArena arena;
-	DevBuf sceneTable, pool[2][4], hitA, hitB, results, images, counters, stats;
+	DevBuf sceneTable, chunkTable, pool[2][4], hitA, hitB, results, images, counters, stats;
 	hipGraphExec_t graph = nullptr;
 	std::vector<hipEvent_t> stepEvents;
-	uint32_t totalPaths = 0, totalPixels = 0;
-	uint32_t *hostCounters = nullptr;    // pinned
+	uint64_t totalPaths = 0, totalPixels = 0;
+	int maxSpp = 1;
+	std::vector<WfChunk> chunks;
+	uint32_t chunkPaths = 0;             // result slots per buffer of the current plan
+	uint32_t *hostCounters = nullptr;    // pinned: counter block read back after each graph launch
 	mrad_batch_stats last{};
 };
 
@@ -653,18 +734,56 @@ void dropGraph(mrad_batch *b) {
 	b->graph = nullptr;
 }
 
-// Uploads the arena and scene table and sizes the pools for the current scenes.
+// Result slots per buffer for a configuration: the requested number, or 8
+// pools' worth clamped to [2^24, 2^30]; never fewer than one pixel's samples.
+uint32_t chunkPathsFor(const mrad_wavefront_config &c, int maxSpp) {
+	uint64_t n = c.result_chunk_paths > 0 ? (uint64_t) c.result_chunk_paths
+	                                      : std::min<uint64_t>(1ull << 30, std::max<uint64_t>(1ull << 24, 8ull * c.queue_capacity));
+	return (uint32_t) std::max<uint64_t>(n, (uint64_t) maxSpp);
+}
+
+// Cuts the batch into chunks of whole pixels of at most chunkPaths paths each.
+void planChunks(mrad_batch *b) {
+	const uint32_t cp = b->chunkPaths;
+	b->chunks.clear();
+	WfChunk cur{0, 0, 0, 0};
+	for (const WfScene &V : b->scenes) {
+		const uint64_t npx = (uint64_t) V.width * (uint64_t) V.height;
+		uint64_t px = 0;
+		while (px < npx) {
+			const uint64_t room = (cp - cur.pathCount) / (uint32_t) V.spp;
+			if (room == 0) {
+				b->chunks.push_back(cur);
+				cur = WfChunk{cur.pathStart + cur.pathCount, cur.pixelStart + cur.pixelCount, 0, 0};
+				continue;
+			}
+			const uint64_t take = std::min(room, npx - px);
+			cur.pathCount += (uint32_t) (take * (uint32_t) V.spp);
+			cur.pixelCount += (uint32_t) take;
+			px += take;
+		}
+	}
+	if (cur.pathCount > 0) b->chunks.push_back(cur);
+}
+
+// Uploads the arena, scene and chunk tables and sizes the pools and result
+// buffers for the current scenes and configuration.
 int prepare(mrad_batch *b) {
 	if (!b->dirty) return MRAD_OK;
 	dropGraph(b);
 	MRAD_TRY(b->arena.upload(b->stream, b->host));
 	MRAD_TRY(upload(b->stream, b->sceneTable, b->scenes));
+	b->chunkPaths = chunkPathsFor(b->cfg, b->maxSpp);
+	planChunks(b);
+	MRAD_TRY(upload(b->stream, b->chunkTable, b->chunks));
 	const uint32_t cap = (uint32_t) b->cfg.queue_capacity;
 	for (int q = 0; q < 2; q++)
 		for (int k = 0; k < 4; k++) MRAD_TRY(ensure(b->pool[q][k], (size_t) cap * 16));
 	MRAD_TRY(ensure(b->hitA, (size_t) cap * 16));
 	MRAD_TRY(ensure(b->hitB, (size_t) cap * 16));
-	MRAD_TRY(ensure(b->results, (size_t) b->totalPaths * 12));
+	// buffer 0 holds the even chunks, buffer 1 (from slot chunkPaths) the odd ones
+	const size_t slots = b->chunks.size() > 1 ? 2 * (size_t) b->chunkPaths : (size_t) b->chunks[0].pathCount;
+	MRAD_TRY(ensure(b->results, slots * 12));
 	MRAD_TRY(ensure(b->images, (size_t) b->totalPixels * 12));
 	MRAD_TRY(ensure(b->counters, kCounterWords * 4));
 	MRAD_TRY(ensure(b->stats, MRAD_STAT_COUNT * 8));
@@ -689,8 +808,9 @@ WfParams params(mrad_batch *b) {
 	P.results = b->results.as<float>();
 	P.images = b->images.as<float>();
 	P.counters = b->counters.as<uint32_t>();
-	P.totalPaths = b->totalPaths;
-	P.totalPixels = b->totalPixels;
+	P.chunks = b->chunkTable.as<const WfChunk>();
+	P.chunkCount = (uint32_t) b->chunks.size();
+	P.chunkPaths = b->chunkPaths;
 	P.capacity = (uint32_t) b->cfg.queue_capacity;
 	P.stats = b->cfg.collect_metrics ? b->stats.as<unsigned long long>() : nullptr;
 	return P;
@@ -734,6 +854,25 @@ int accumulateStep(mrad_batch *b, hipEvent_t *ev) {
 	return MRAD_OK;
 }
 
+// Defaults and limits of a wavefront configuration (create and configure).
+int normalizeConfig(mrad_wavefront_config &c) {
+	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
+	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
+	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
+	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
+	if (c.lds_stack_entries != 4 && c.lds_stack_entries != kShortStack)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "LDS stack entries must be 4 or " + std::to_string(kShortStack));
+	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28))
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "queue capacity must be in [64, 2^28]");
+	if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return set_error(MRAD_ERR_INVALID_ARGUMENT, "waves per CU must be in [1, 32]");
+	if (c.bvh_stack_size < 8 || c.bvh_stack_size > 256)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH stack size must be in [8, 256]");
+	if (c.result_chunk_paths < 0 || c.result_chunk_paths > (1 << 30))
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "result chunk paths must be in [0, 2^30]");
+	c.queue_capacity = (c.queue_capacity + 63) & ~63;
+	return MRAD_OK;
+}
+
 int buildGraph(mrad_batch *b, const WfParams &P) {
 	if (b->graph) return MRAD_OK;
 	hipGraph_t g = nullptr;
@@ -764,19 +903,7 @@ int mrad_batch_create(const mrad_wavefront_config *config, mrad_batch **out) {
 	if (!config || !out) return set_error(MRAD_ERR_INVALID_ARGUMENT, "config and output handle are required");
 	*out = nullptr;
 	mrad_wavefront_config c = *config;
-	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
-	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
-	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
-	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
-	if (c.lds_stack_entries != 4 && c.lds_stack_entries != kShortStack)
-		return set_error(MRAD_ERR_INVALID_ARGUMENT,
-		                 "LDS stack entries must be 4 or " + std::to_string(kShortStack));
-	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28))
-		return set_error(MRAD_ERR_INVALID_ARGUMENT, "queue capacity must be in [64, 2^28]");
-	if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return set_error(MRAD_ERR_INVALID_ARGUMENT, "waves per CU must be in [1, 32]");
-	if (c.bvh_stack_size < 8 || c.bvh_stack_size > 256)
-		return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH stack size must be in [8, 256]");
-	c.queue_capacity = (c.queue_capacity + 63) & ~63;
+	MRAD_TRY(normalizeConfig(c));
 	mrad_batch *b = new (std::nothrow) mrad_batch();
 	if (!b) return set_error(MRAD_ERR_OUT_OF_MEMORY, "batch allocation failed");
 	b->cfg = c;
@@ -814,7 +941,8 @@ void mrad_batch_destroy(mrad_batch *b) {
 			b->arena.release();
 			for (int q = 0; q < 2; q++)
 				for (int k = 0; k < 4; k++) b->pool[q][k].release();
-			for (DevBuf *x : {&b->sceneTable, &b->hitA, &b->hitB, &b->results, &b->images, &b->counters, &b->stats})
+			for (DevBuf *x : {&b->sceneTable, &b->chunkTable, &b->hitA, &b->hitB, &b->results, &b->images, &b->counters,
+			                  &b->stats})
 				x->release();
 			if (b->hostCounters) (void) hipHostFree(b->hostCounters);
 			for (hipEvent_t e : b->stepEvents) (void) hipEventDestroy(e);
@@ -833,12 +961,14 @@ int mrad_batch_add_scene(mrad_batch *b, const mrad_scene_arrays *a, const mrad_v
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "view width, height and samples must be positive");
 	MRAD_TRY(mrad_validate_full_path_depth(v->max_depth));
 	if (v->rr_start < -1) return set_error(MRAD_ERR_INVALID_ARGUMENT, "roulette start depth must be >= -1");
+	// a scene's pixel index is 32-bit (the reference's y * width + x); paths of
+	// the whole batch are numbered with 64 bits
+	const uint64_t pixels = (uint64_t) v->width * v->height;
+	if (pixels > 0xFFFFFFFFull) return set_error(MRAD_ERR_INVALID_ARGUMENT, "view exceeds 2^32 pixels");
+	if (v->samples > (1 << 30)) return set_error(MRAD_ERR_INVALID_ARGUMENT, "view samples exceed 2^30");
+	const uint64_t paths = pixels * (uint64_t) v->samples;
 	std::lock_guard<std::mutex> g(b->mu);
 	if (b->scenes.size() >= 0xFFFF) return set_error(MRAD_ERR_INVALID_ARGUMENT, "at most 65535 scenes per batch");
-	const uint64_t pixels = (uint64_t) v->width * v->height;
-	const uint64_t paths = pixels * (uint64_t) v->samples;
-	if ((uint64_t) b->totalPaths + paths >= 0xFFFFFFFFull || (uint64_t) b->totalPixels + pixels >= 0x7FFFFFFFull)
-		return set_error(MRAD_ERR_INVALID_ARGUMENT, "batch exceeds 2^32 paths");
 	WfScene s{};
 	MRAD_TRY(b->host.append(a, b->cfg.linear_scan == 0, b->cfg.bvh_stack_size, s.R));
 	std::memcpy(s.cam, v->camera, sizeof s.cam);
@@ -851,8 +981,9 @@ int mrad_batch_add_scene(mrad_batch *b, const mrad_scene_arrays *a, const mrad_v
 	s.frameSeed = v->frame_seed;
 	s.pathBase = b->totalPaths;
 	s.pixelBase = b->totalPixels;
-	b->totalPaths += (uint32_t) paths;
-	b->totalPixels += (uint32_t) pixels;
+	b->totalPaths += paths;
+	b->totalPixels += pixels;
+	b->maxSpp = std::max(b->maxSpp, v->samples);
 	b->scenes.push_back(s);
 	b->dirty = true;
 	if (index) *index = (int) b->scenes.size() - 1;
@@ -878,12 +1009,15 @@ int mrad_batch_render(mrad_batch *b) {
 		MRAD_HIP(hipEventCreate(&e));
 		b->stepEvents.push_back(e);
 	}
-	// counters: zero in host memory, then one copy (no memset node is recorded)
-	std::memset(b->hostCounters, 0, kCounterWords * 4);
-	MRAD_HIP(hipMemcpyAsync(b->counters.p, b->hostCounters, kCounterWords * 4, hipMemcpyHostToDevice, b->stream));
+	// counters: zero, chunks 0 and 1 released (a kernel, not a memset or copy)
+	const uint32_t chunkCount = (uint32_t) b->chunks.size();
+	hipLaunchKernelGGL(wfInitCounters, dim3(1), dim3(64), 0, b->stream, P);
+	MRAD_HIP(hipGetLastError());
 	if (b->cfg.collect_metrics) MRAD_HIP(hipMemsetAsync(b->stats.p, 0, MRAD_STAT_COUNT * 8, b->stream));
 	MRAD_HIP(hipEventRecord(b->evStart, b->stream));
+	uint32_t resolved = 0;        // chunks resolved so far, in order
 	int64_t steps = 0;
+	const int64_t stepGuard = 64 + 4 * 33 * ((int64_t) (b->totalPaths / 64) + chunkCount + 1);
 	for (;;) {
 		if (timed) {
 			for (int cur = 0; cur < 2; cur++) {
@@ -899,14 +1033,38 @@ int mrad_batch_render(mrad_batch *b) {
 		steps += 2;
 		MRAD_HIP(hipMemcpyAsync(b->hostCounters, b->counters.p, kCounterWords * 4, hipMemcpyDeviceToHost, b->stream));
 		MRAD_HIP(hipStreamSynchronize(b->stream));
+		// Every chunk whose paths have all finished is resolved, in order; its
+		// buffer's count is reset and the chunk two ahead may then be generated
+		// into it.  Both kernels are stream-ordered before the next graph launch.
+		// Only chunks released before this snapshot can be complete: the count
+		// word of chunk r + 2 still holds chunk r's count until the release.
 		const uint32_t *c = b->hostCounters;
-		if (c[kNextPath] >= b->totalPaths && c[kCount0] == 0 && c[kCount1] == 0) break;
-		if (steps > (int64_t) 64 + 4 * ((int64_t) b->totalPaths / 64 + 1) * 33)
-			return set_error(MRAD_ERR_ILLEGAL_STATE, "wavefront did not drain");
+		const uint32_t before = resolved;
+		while (resolved < chunkCount && resolved < c[kGenLimit] &&
+		       c[kDone0 + (resolved & 1u)] == b->chunks[resolved].pathCount) {
+			const uint32_t px = b->chunks[resolved].pixelCount;
+			const int blocks = std::max(1, std::min(16384, (int) ((px + kBlock - 1) / kBlock)));
+			hipLaunchKernelGGL(wfResolve, dim3(blocks), dim3(kBlock), 0, b->stream, P, resolved);
+			MRAD_HIP(hipGetLastError());
+			resolved++;
+		}
+		if (resolved == chunkCount) break;
+		if (resolved != before) {
+			hipLaunchKernelGGL(wfRelease, dim3(1), dim3(64), 0, b->stream, P, before, resolved);
+			MRAD_HIP(hipGetLastError());
+		}
+		if (steps > stepGuard) {
+			const uint32_t r = resolved;
+			return set_error(MRAD_ERR_ILLEGAL_STATE,
+			                 "wavefront did not drain: " + std::to_string(r) + " of " + std::to_string(chunkCount) +
+			                     " chunks resolved (chunk " + std::to_string(r) + ": " +
+			                     std::to_string(b->chunks[r].pathCount) + " paths, " +
+			                     std::to_string(c[kDone0 + (r & 1u)]) + " finished); pools " + std::to_string(c[kCount0]) +
+			                     " / " + std::to_string(c[kCount1]) + ", generating chunk " + std::to_string(c[kGenChunk]) +
+			                     " at " + std::to_string(c[kGenOffset]) + " (limit " + std::to_string(c[kGenLimit]) +
+			                     "), finished " + std::to_string(c[kDone0]) + " / " + std::to_string(c[kDone1]));
+		}
 	}
-	const int resolveBlocks = std::max(1, std::min(16384, (int) ((b->totalPixels + kBlock - 1) / kBlock)));
-	hipLaunchKernelGGL(wfResolve, dim3(resolveBlocks), dim3(kBlock), 0, b->stream, P);
-	MRAD_HIP(hipGetLastError());
 	MRAD_HIP(hipEventRecord(b->evEnd, b->stream));
 	MRAD_HIP(hipEventSynchronize(b->evEnd));
 	float ms = 0.0f;
@@ -928,18 +1086,13 @@ int mrad_batch_configure(mrad_batch *b, const mrad_wavefront_config *config) {
 	MRAD_TRY(requireBatch(b));
 	if (!config) return set_error(MRAD_ERR_INVALID_ARGUMENT, "config is required");
 	mrad_wavefront_config c = *config;
-	if (c.queue_capacity == 0) c.queue_capacity = 1 << 22;
-	if (c.waves_per_cu == 0) c.waves_per_cu = 28;
-	if (c.bvh_stack_size == 0) c.bvh_stack_size = 32;
-	if (c.lds_stack_entries == 0) c.lds_stack_entries = kShortStack;
-	if (c.queue_capacity < 64 || c.queue_capacity > (1 << 28) || c.waves_per_cu < 1 || c.waves_per_cu > 32 ||
-	    (c.lds_stack_entries != 4 && c.lds_stack_entries != kShortStack))
-		return set_error(MRAD_ERR_INVALID_ARGUMENT, "invalid wavefront configuration");
+	MRAD_TRY(normalizeConfig(c));
 	std::lock_guard<std::mutex> g(b->mu);
 	if (c.device != b->cfg.device || c.bvh_stack_size != b->cfg.bvh_stack_size || c.linear_scan != b->cfg.linear_scan)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "device, stack size and traversal mode are fixed per batch");
-	c.queue_capacity = (c.queue_capacity + 63) & ~63;
 	MRAD_HIP(hipSetDevice(b->cfg.device));
+	// a new chunk size re-plans the chunks and result buffers on the next render
+	if (chunkPathsFor(c, b->maxSpp) != chunkPathsFor(b->cfg, b->maxSpp)) b->dirty = true;
 	dropGraph(b);
 	const bool grow = (size_t) c.queue_capacity * 16 > b->pool[0][0].cap;
 	if (grow && !b->dirty) {

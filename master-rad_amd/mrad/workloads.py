@@ -359,6 +359,50 @@ def hero_gpu_camera(width: int = WIDTH, height: int = HEIGHT) -> GpuCamera:
     return gpu_camera(hero_camera(), width, height)
 
 
+# ---- GI Test (graphics3d/scene/v1/GITestPlain.java:15-49) and its catalog camera
+GI_TEST_WIDTH, GI_TEST_HEIGHT = 1280, 720       # the platform comparison's frame (921,600 camera samples)
+
+
+def _hsb(h: float, s: float, b: float):
+    """Color.hsb (Color.java:57-74) for the hues GI Test uses."""
+    h = h - math.floor(h)
+    base = int(h * 6.0)
+    f = h * 6.0 - base
+    p, q, t = b * (1.0 - s), b * (1.0 - s * f), b * (1.0 - s * (1.0 - f))
+    return [(b, t, p), (q, b, p), (p, b, t), (p, q, b), (t, p, b), (b, p, q)][base]
+
+
+def gi_test_scene(config: BvhBuildConfig | None = None) -> GpuScene:
+    """GITestPlain: five half-spaces (matte side and back walls, an emissive ceiling), a glass sphere, a mirror
+    sphere, a matte sphere and a partly reflective sphere; black background.  Materials are
+    SceneSupport.matte / emissive / surface (SceneSupport.java:16-37, Material.java:86-92)."""
+    def m(emission=(0.0, 0.0, 0.0), diffuse=(0.0, 0.0, 0.0), reflective=(0.0, 0.0, 0.0), refractive=(0.0, 0.0, 0.0),
+          ior=1.5):
+        return material_array(diffuse, reflective, refractive, emission, ior)[0]
+
+    grey = (0.7, 0.7, 0.7)
+    sc = GpuScene(config or BvhBuildConfig.uniform(LEAF_SIZE))
+    sc.set_background((0.0, 0.0, 0.0))
+    spheres = [[-0.2, -0.5, 0.0, 0.3], [0.5, -0.5, -0.3, 0.3], [0.0, 0.2, 0.0, 0.2], [-0.4, 0.5, 0.1, 0.2]]
+    smats = [m(reflective=(0.05,) * 3, refractive=(0.95,) * 3, ior=1.4), m(reflective=(1.0,) * 3),
+             m(diffuse=grey), m(reflective=(0.9,) * 3)]
+    sc.add_spheres(spheres, np.array(smats))
+    planes = [[-1, 0, 0, 1, 0, 0], [1, 0, 0, -1, 0, 0], [0, -1, 0, 0, 1, 0], [0, 1, 0, 0, -1, 0], [0, 0, 1, 0, 0, -1]]
+    pmats = [m(diffuse=_hsb(0.0, 0.5, 0.7)), m(diffuse=_hsb(1.0 / 3.0, 0.5, 0.7)), m(diffuse=grey),
+             m(emission=(1.0, 1.0, 1.0)), m(diffuse=grey)]
+    sc.add_planes(planes, np.array(pmats))
+    return sc
+
+
+def gi_test_camera() -> Camera:
+    """SceneCatalog.create(GI_TEST) (playground/SceneCatalog.java:35-37): SceneSupport.camera((0, 0, -4), 0, 0)."""
+    return support_camera((0.0, 0.0, -4.0), 0.0, 0.0)
+
+
+def gi_test_gpu_camera(width: int = GI_TEST_WIDTH, height: int = GI_TEST_HEIGHT) -> GpuCamera:
+    return gpu_camera(gi_test_camera(), width, height)
+
+
 def scale_id(scale: int) -> str:
     """BenchmarkProtocol.compact (:410-417) as used in the row ids."""
     return "hero-" + {1_000: "1k", 10_000: "10k", 100_000: "100k", 1_000_000: "1m"}.get(scale, str(scale))

@@ -141,3 +141,19 @@ def test_scene_with_bad_material_index_is_rejected_on_the_host(make):
             assert rc == N.MRAD_ERR_INVALID_ARGUMENT, label
             assert b"material index" in N.lib.mrad_last_error(), label
         assert b.add_scene(sc, cam, 8, 8, 1, 4) == 0    # the valid scene is still accepted, as the first
+
+
+def test_batch_accepts_more_than_2_32_paths_and_checks_view_limits():
+    """Paths of a batch are numbered with 64 bits: views totalling more than 2^32 paths are accepted (host logic,
+    no device needed).  A view of more than 2^32 pixels or 2^30 samples is an invalid argument."""
+    sc, cam = small_field(200, 4), configs.field_camera()
+    with mrad.BatchRenderer(mrad.WavefrontConfig()) as b:
+        for _ in range(3):
+            b.add_scene(sc, cam, 2048, 2048, 512, 4)          # 2.1e9 paths each
+        for w, h, spp in ((1 << 17, 1 << 16, 1), (8, 8, (1 << 30) + 1)):
+            view = mrad.batch.make_view(cam, w, h, spp, 4, 0, 1)
+            rc = N.lib.mrad_batch_add_scene(b._h, C.byref(sc.arrays()), C.byref(view), None)
+            assert rc == N.MRAD_ERR_INVALID_ARGUMENT
+    for bad in (-1, (1 << 30) + 1):
+        with pytest.raises(mrad.InvalidArgument):
+            mrad.BatchRenderer(mrad.WavefrontConfig(result_chunk_paths=bad))

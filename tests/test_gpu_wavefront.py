@@ -116,3 +116,48 @@ def test_batch_rerender_is_deterministic(require_gpu):
         first = b.image(0)
         b.render()
         np.testing.assert_array_equal(first, b.image(0))
+
+
+@pytest.mark.parametrize("chunk,capacity", [(4096, 2048), (2999, 1 << 20)])
+def test_batch_small_result_chunks_equal_megakernel(require_gpu, chunk, capacity):
+    """Result chunks of a few thousand paths (dozens of chunks of whole pixels, the two result buffers taking
+    turns, generation held back until a buffer is resolved) give the megakernel's images bit for bit, also with
+    a zero-depth scene in the batch and chunk sizes that are no multiple of the sample counts."""
+    cfg = mrad.WavefrontConfig(queue_capacity=capacity, result_chunk_paths=chunk, waves_per_cu=8)
+    refs = []
+    with mrad.BatchRenderer(cfg) as b:
+        for k, (mk_scene, mk_cam, w, h, spp, depth, rr, frame) in enumerate(CASES):
+            sc, cam = mk_scene(), mk_cam()
+            seed = mrad.frame_seed(0xC4A + k, frame)
+            b.add_scene(sc, cam, w, h, spp, depth, frame, seed, rr)
+            refs.append(_mega(sc, cam, w, h, spp, depth, frame, seed, rr)[0])
+        sc, cam = configs.room_scene(), configs.room_camera()
+        b.add_scene(sc, cam, 20, 10, 3, 0, 0, 5)
+        print(f"batch result chunks {chunk}, pool {capacity}", flush=True)
+        st = b.render()
+        assert st["paths"] == sum(c[2] * c[3] * c[4] for c in CASES) + 600
+        for k, ref in enumerate(refs):
+            np.testing.assert_array_equal(b.image(k).view(np.uint32), ref.view(np.uint32))
+        assert np.all(b.image(len(refs)) == 0)
+        # a different chunk size on the same batch re-plans and renders the same images
+        b.configure(mrad.WavefrontConfig(queue_capacity=capacity, result_chunk_paths=0, waves_per_cu=8))
+        b.render()
+        for k, ref in enumerate(refs):
+            np.testing.assert_array_equal(b.image(k).view(np.uint32), ref.view(np.uint32))
+
+
+def test_batch_of_more_than_2_32_paths(require_gpu):
+    """Two 1024x1024 views of 2100 samples each (4.4e9 paths, more than 2^32) render in one batch, each image
+    equal to the megakernel's frame bit for bit."""
+    sc, cam = small_field(1000, 9), configs.field_camera()
+    w, h, spp, depth = 1024, 1024, 2100, 2
+    seeds = [mrad.frame_seed(0xB16, k) for k in range(2)]
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 24)) as b:
+        for k in range(2):
+            b.add_scene(sc, cam, w, h, spp, depth, k, seeds[k])
+        print(f"batch of {2 * w * h * spp} paths", flush=True)
+        st = b.render()
+        assert st["paths"] == 2 * w * h * spp > 1 << 32
+        for k in range(2):
+            ref = _mega(sc, cam, w, h, spp, depth, k, seeds[k])[0]
+            np.testing.assert_array_equal(b.image(k).view(np.uint32), ref.view(np.uint32))
