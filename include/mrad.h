@@ -249,6 +249,8 @@ typedef struct {
 	int leaf_regroup;        /* 1 = regroup each wave's leaf tests by primitive class */
 	int result_chunk_paths;  /* paths of whole pixels whose radiance is held for the ordered per-pixel sum, per
 	                            result buffer (two are kept): 0 = 8 pools' worth in [2^24, 2^30]; at most 2^30 */
+	int stack_storage;       /* traversal-stack entries per lane compiled ahead of time: 32, 64 or 256, at least
+	                            bvh_stack_size (0 = 64, or 256 above a stack size of 64); the autotuned depth */
 } mrad_wavefront_config;
 
 typedef struct {

@@ -682,15 +682,23 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 	if (P.stats) flushCounters<kMetrics>(c, P.stats);
 }
 
-// Trace kernels compiled ahead of time: stack storage 64 or 256 entries, of
-// which 4 or 8 are held in LDS (the autotuned LDS split); optionally with
-// primitive-class regrouping of the leaf tests.
+// Trace kernels compiled ahead of time: stack storage of 32, 64 or 256
+// entries per lane (the autotuned stack depth; at least the runtime limit),
+// of which 4 or 10 are held in LDS (the autotuned LDS split); optionally
+// with primitive-class regrouping of the leaf tests.
 using TraceFn = void (*)(WfParams, int);
+// Storage entries for a configuration: the requested size, else 64 (256
+// above a limit of 64).
+inline int stackStorage(const mrad_wavefront_config &c) {
+	return c.stack_storage > 0 ? c.stack_storage : (c.bvh_stack_size <= 64 ? 64 : 256);
+}
 template <bool M>
-TraceFn traceFor(int cap, int ldsEntries, bool regroup) {
-	if (regroup) return cap <= 64 ? wfTraceRegroup<M, 64, 8> : wfTraceRegroup<M, 256, 8>;
-	if (ldsEntries <= 4) return cap <= 64 ? wfTrace<M, 64, 4> : wfTrace<M, 256, 4>;
-	return cap <= 64 ? wfTrace<M, 64, kShortStack> : wfTrace<M, 256, kShortStack>;
+TraceFn traceFor(int storage, int ldsEntries, bool regroup) {
+	if (regroup) return storage <= 64 ? wfTraceRegroup<M, 64, 8> : wfTraceRegroup<M, 256, 8>;
+	if (ldsEntries <= 4)
+		return storage <= 32 ? wfTrace<M, 32, 4> : storage <= 64 ? wfTrace<M, 64, 4> : wfTrace<M, 256, 4>;
+	return storage <= 32 ? wfTrace<M, 32, kShortStack>
+	     : storage <= 64 ? wfTrace<M, 64, kShortStack> : wfTrace<M, 256, kShortStack>;
 }
 
 }  // namespace mrad
@@ -830,8 +838,8 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 	if (ev) MRAD_HIP(hipEventRecord(ev[1], b->stream));
 	const bool regroup = b->cfg.leaf_regroup != 0;
 	const TraceFn trace = b->cfg.collect_metrics
-	                          ? traceFor<true>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries, regroup)
-	                          : traceFor<false>(b->cfg.bvh_stack_size, b->cfg.lds_stack_entries, regroup);
+	                          ? traceFor<true>(stackStorage(b->cfg), b->cfg.lds_stack_entries, regroup)
+	                          : traceFor<false>(stackStorage(b->cfg), b->cfg.lds_stack_entries, regroup);
 	hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
 	hipLaunchKernelGGL(wfShade, dim3(shadeBlocks), dim3(kBlock), 0, b->stream, P, cur);
@@ -867,6 +875,10 @@ int normalizeConfig(mrad_wavefront_config &c) {
 	if (c.waves_per_cu < 1 || c.waves_per_cu > 32) return set_error(MRAD_ERR_INVALID_ARGUMENT, "waves per CU must be in [1, 32]");
 	if (c.bvh_stack_size < 8 || c.bvh_stack_size > 256)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH stack size must be in [8, 256]");
+	if (c.stack_storage != 0 && c.stack_storage != 32 && c.stack_storage != 64 && c.stack_storage != 256)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "stack storage must be 0, 32, 64 or 256 entries");
+	if (c.stack_storage != 0 && c.stack_storage < c.bvh_stack_size)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "stack storage must hold the BVH stack size");
 	if (c.result_chunk_paths < 0 || c.result_chunk_paths > (1 << 30))
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "result chunk paths must be in [0, 2^30]");
 	c.queue_capacity = (c.queue_capacity + 63) & ~63;

@@ -1,11 +1,13 @@
 """Autotuner of the wavefront form's launch parameters.
 
 Coordinate descent over queue capacity (paths in flight), waves per CU of
-the persistent kernels and the LDS split of the traversal stack (entries per
-lane in LDS), on a loaded batch: each candidate is applied with
+the persistent kernels, the LDS split of the traversal stack (entries per
+lane in LDS) and the stack depth (entries per lane of the ahead-of-time
+stack storage), on a loaded batch: each candidate is applied with
 mrad_batch_configure (scenes stay resident), rendered once to warm and then
 timed.  Images do not depend on these parameters (every path runs the same
-arithmetic), so only time is compared.
+arithmetic), so only time is compared.  A time budget bounds the search: once
+it is spent, the best configuration so far is kept.
 """
 from __future__ import annotations
 
@@ -18,6 +20,7 @@ GRID = {
     "queue_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26],
     "waves_per_cu": [16, 24, 28, 32],
     "lds_stack_entries": [4, 10],
+    "stack_storage": [32, 64],
 }
 
 
@@ -33,12 +36,15 @@ def _time(batch: BatchRenderer, cfg: WavefrontConfig, repeats: int) -> float:
 
 
 def tune(batch: BatchRenderer, start: WavefrontConfig | None = None, grid: dict | None = None, passes: int = 1,
-         repeats: int = 1, log=None) -> tuple[WavefrontConfig, list[dict]]:
-    """Returns the fastest configuration found and the table of every trial."""
+         repeats: int = 1, log=None, budget_s: float | None = None) -> tuple[WavefrontConfig, list[dict]]:
+    """Returns the fastest configuration found and the table of every trial.
+    With `budget_s`, no new candidate is started once that many seconds have
+    passed."""
     grid = grid or GRID
     best = dataclasses.replace(start or batch.config, use_graph=True, time_kernels=False, collect_metrics=False)
     seen: dict[tuple, float] = {}
     table = []
+    t0 = time.perf_counter()
 
     def trial(cfg):
         key = tuple(getattr(cfg, k) for k in grid)
@@ -53,6 +59,9 @@ def tune(batch: BatchRenderer, start: WavefrontConfig | None = None, grid: dict 
     for _ in range(passes):
         for name, values in grid.items():
             for v in values:
+                if budget_s is not None and time.perf_counter() - t0 > budget_s:
+                    batch.configure(best)
+                    return best, table
                 cand = dataclasses.replace(best, **{name: v})
                 t = trial(cand)
                 if t < best_t:

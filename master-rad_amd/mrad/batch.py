@@ -28,13 +28,15 @@ class WavefrontConfig:
     lds_stack_entries: int = 10
     leaf_regroup: bool = False
     result_chunk_paths: int = 0
+    stack_storage: int = 0
     device: int = 0
 
     def native(self) -> N.WavefrontConfig:
         return N.WavefrontConfig(self.device, self.queue_capacity, self.waves_per_cu, self.bvh_stack_size,
                                  1 if self.linear_scan else 0, 1 if self.collect_metrics else 0,
                                  1 if self.use_graph else 0, 1 if self.time_kernels else 0,
-                                 self.lds_stack_entries, 1 if self.leaf_regroup else 0, self.result_chunk_paths)
+                                 self.lds_stack_entries, 1 if self.leaf_regroup else 0, self.result_chunk_paths,
+                                 self.stack_storage)
 
 
 def make_view(camera, width: int, height: int, samples: int, max_depth: int, frame_index: int, frame_seed: int,

@@ -10,6 +10,12 @@
 
 Run from master-rad_amd/ (or with it on sys.path).  Each prints one line per
 step so a profiled run shows progress.
+
+    python -m mrad.profile_run summarize OUT.json KERNEL_REGEX CSV [CSV ...]
+
+averages the counters of every dispatch whose kernel name matches
+KERNEL_REGEX over the rocprofv3 counter_collection.csv files given (one file
+per counter pass) and writes them, per dispatch, to OUT.json.
 """
 from __future__ import annotations
 
@@ -100,8 +106,40 @@ def _random(frames: int = 5):
             print(f"F4-10k frame {i}: {t.last_frame_stats()['kernel_nanos'] / 1e6:.3f} ms", flush=True)
 
 
+def summarize(out: str, kernel_regex: str, csvs: list[str]) -> dict:
+    """Per-dispatch means of rocprofv3 counters (counter_collection.csv rows:
+    one per dispatch, counter and, for per-XCD counters, instance) of the
+    dispatches whose kernel name matches kernel_regex."""
+    import csv
+    import json
+    import re
+    pat = re.compile(kernel_regex)
+    totals: dict[str, dict[str, float]] = {}
+    kernels = set()
+    for path in csvs:
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                name = row.get("Kernel_Name", "")
+                if not pat.search(name):
+                    continue
+                kernels.add(name)
+                key = f"{path}:{row.get('Dispatch_Id', row.get('Correlation_Id', ''))}"
+                d = totals.setdefault(row["Counter_Name"], {})
+                d[key] = d.get(key, 0.0) + float(row["Counter_Value"])
+    res = {"kernel_regex": kernel_regex, "kernels": sorted(kernels), "files": csvs,
+           "dispatches": {c: len(v) for c, v in totals.items()},
+           "per_dispatch": {c: sum(v.values()) / len(v) for c, v in totals.items()}}
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res["per_dispatch"]), flush=True)
+    return res
+
+
 def main(argv):
     what = argv[1] if len(argv) > 1 else "het"
+    if what == "summarize":
+        summarize(argv[2], argv[3], argv[4:])
+        return
     if what == "het":
         _het(int(argv[2]) if len(argv) > 2 else 16)
     elif what == "batch":

@@ -60,9 +60,17 @@ def _lib() -> C.CDLL:
     lib.or64_render.restype = C.c_int
     lib.or64_render.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    lib.or64_render_window.restype = C.c_int
+    lib.or64_render_window.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.c_int]
     lib.or_adjoint.restype = C.c_int
     lib.or_adjoint.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
-                               C.c_int, fp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+                               C.c_int, fp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    lib.or_record_paths.restype = C.c_int
+    lib.or_record_paths.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,
+                                    C.c_int, ip, C.POINTER(C.c_uint8), fp, C.POINTER(C.c_uint8),
+                                    C.POINTER(C.c_uint8)]
     lib.or_render_fixed_paths64.restype = C.c_int
     lib.or_render_fixed_paths64.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                             C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -179,18 +187,106 @@ def render64(scene: Scene, camera: np.ndarray, width: int, height: int, max_dept
     return (mean, var) if variance else mean
 
 
+def render64_window(scene: Scene, camera: np.ndarray, width: int, height: int, window, max_depth: int, spp: int,
+                    seed: int, rr_start: int = -1, threads: int | None = None):
+    """FP64 mean and per-pixel sample variance ((ch*cw, 3) each) of the window (x0, y0, cw, ch) of a
+    width x height frame: the same pixels, camera mapping and random numbers as render64's full frame."""
+    x0, y0, cw, ch = window
+    cam = np.ascontiguousarray(camera, np.float32)
+    mean = np.zeros((ch * cw, 3), np.float64)
+    var = np.zeros((ch * cw, 3), np.float64)
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.or64_render_window(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, x0, y0, cw, ch, max_depth,
+                                rr_start, spp, seed & ((1 << 64) - 1), mean.ctypes.data_as(C.POINTER(C.c_double)),
+                                var.ctypes.data_as(C.POINTER(C.c_double)), threads)
+    assert rc == 0
+    return mean, var
+
+
 def adjoint(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, frame_index: int,
-            frame_seed: int, spp: int, grad_image: np.ndarray, primitive_count: int, rr_start: int = -1):
+            frame_seed: int, spp: int, grad_image: np.ndarray, primitive_count: int, rr_start: int = -1,
+            threads: int | None = None):
     """Path-replay adjoint with FP32 paths and FP64 sums: (grad_albedo, grad_emission) per primitive."""
     cam = np.ascontiguousarray(camera, np.float32)
     g = np.ascontiguousarray(grad_image, np.float32).reshape(-1)
     ga = np.zeros(primitive_count * 3, np.float64)
     ge = np.zeros(primitive_count * 3, np.float64)
+    threads = threads or min(16, os.cpu_count() or 1)
     rc = lib.or_adjoint(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, max_depth, rr_start, frame_index,
                         frame_seed & ((1 << 64) - 1), spp, g.ctypes.data_as(fp),
-                        ga.ctypes.data_as(C.POINTER(C.c_double)), ge.ctypes.data_as(C.POINTER(C.c_double)))
+                        ga.ctypes.data_as(C.POINTER(C.c_double)), ge.ctypes.data_as(C.POINTER(C.c_double)), threads)
     assert rc == 0
     return ga.reshape(-1, 3), ge.reshape(-1, 3)
+
+
+class RecordedPaths:
+    """The vertices of a frame's paths (or_record_paths): for the FP64 re-evaluation of the frame's estimator
+    under other material parameters, an exact polynomial in albedo and emission for Lambertian materials."""
+
+    def __init__(self, scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, frame_index: int,
+                 frame_seed: int, spp: int, threads: int | None = None):
+        n = width * height * spp
+        self.width, self.height, self.spp, self.depth = width, height, spp, max(1, max_depth)
+        self.mat = np.full((n, self.depth), -1, np.int32)
+        self.lobe = np.zeros((n, self.depth), np.uint8)
+        self.prob = np.zeros((n, self.depth), np.float32)
+        self.count = np.zeros(n, np.uint8)
+        self.end = np.zeros(n, np.uint8)
+        self.background = np.asarray(scene.w["background"], np.float64).reshape(3)
+        cam = np.ascontiguousarray(camera, np.float32)
+        threads = threads or min(16, os.cpu_count() or 1)
+        u8 = C.POINTER(C.c_uint8)
+        rc = lib.or_record_paths(C.byref(scene.c), cam.ctypes.data_as(fp), width, height, max_depth, frame_index,
+                                 frame_seed & ((1 << 64) - 1), spp, threads, self.mat.ctypes.data_as(ip),
+                                 self.lobe.ctypes.data_as(u8), self.prob.ctypes.data_as(fp),
+                                 self.count.ctypes.data_as(u8), self.end.ctypes.data_as(u8))
+        assert rc == 0
+        self._index = None
+
+    def radiance(self, materials64: np.ndarray, paths=None) -> np.ndarray:
+        """FP64 radiance (k, 3) of the given paths (all when None) under the material table materials64 (m x 13):
+        sum over vertices of T_i Le_i with T_{i+1} = T_i c_i[lobe] / p_i, plus T_n times the background when the
+        last ray missed (or_render_fixed_paths64's arithmetic)."""
+        sel = slice(None) if paths is None else paths
+        mat, lobe, prob = self.mat[sel], self.lobe[sel], self.prob[sel]
+        count, end = self.count[sel], self.end[sel]
+        flat = np.ascontiguousarray(materials64, np.float64).reshape(-1)
+        k = len(count)
+        T = np.ones((k, 3))
+        L = np.zeros((k, 3))
+        for i in range(self.depth):
+            live = np.flatnonzero(count > i)
+            if len(live) == 0:
+                break
+            mi = mat[live, i].astype(np.int64)
+            ok = mi >= 0
+            base = np.where(ok, mi, 0) * 13
+            Tl = T[live]
+            le = np.stack([flat[base + 9], flat[base + 10], flat[base + 11]], 1)
+            L[live] += np.where(ok[:, None], Tl * le, 0.0)
+            # every live vertex scatters but one at which the path was absorbed
+            scat = ok & ~((end[live] == 2) & (count[live] == i + 1))
+            col = base + 3 * lobe[live, i].astype(np.int64)
+            p = prob[live, i].astype(np.float64)
+            inv = np.where(p > 0, 1.0 / np.where(p > 0, p, 1.0), 0.0)
+            f = np.stack([flat[col], flat[col + 1], flat[col + 2]], 1) * inv[:, None]
+            T[live] = np.where(scat[:, None], Tl * f, np.where(ok[:, None], Tl, 0.0))
+        L += np.where((end == 1)[:, None], T * self.background[None, :], 0.0)
+        return L
+
+    def image(self, materials64: np.ndarray) -> np.ndarray:
+        """The frame (W*H*3 doubles): each pixel the mean of its spp path radiances."""
+        return self.radiance(materials64).reshape(-1, self.spp, 3).mean(axis=1).reshape(-1)
+
+    def paths_of_material(self, m: int) -> np.ndarray:
+        """Indices of the paths with a vertex on material m."""
+        if self._index is None:
+            flat = self.mat.reshape(-1)
+            order = np.argsort(flat, kind="stable")
+            self._index = (flat[order], order // self.depth)
+        keys, paths = self._index
+        lo, hi = np.searchsorted(keys, m, "left"), np.searchsorted(keys, m, "right")
+        return np.unique(paths[lo:hi])
 
 
 def render_fixed_paths64(scene: Scene, camera: np.ndarray, width: int, height: int, max_depth: int, frame_index: int,

@@ -868,22 +868,70 @@ static v3 adjointPath(const OrScene *s, v3 o, v3 d, int maxDepth, int rrStart, u
 }
 
 /* Adjoint of the whole frame: delta = dLoss/dImage (W*H*3 floats); gradients
- * per primitive in the reference's global order (grad arrays nPrim x 3). */
-int or_adjoint(const OrScene *s, const float *cam, int width, int height, int maxDepth, int rrStart, int frameIndex,
-		uint64_t frameSeed, int spp, const float *gradImage, double *gAlb, double *gEmit) {
-	if (!s || !cam || !gradImage || !gAlb || !gEmit || spp < 1 || maxDepth > 32) return -1;
-	for (int y = 0; y < height; y++)
-		for (int x = 0; x < width; x++) {
-			uint32_t pixel = (uint32_t) y * (uint32_t) width + (uint32_t) x;
+ * per primitive in the reference's global order (grad arrays nPrim x 3).
+ * Rows are split over `threads` threads, each summing into its own arrays;
+ * the per-thread sums are added in thread order. */
+typedef struct {
+	const OrScene *s; const float *cam; int width, height, maxDepth, rrStart, frameIndex, spp; uint64_t frameSeed;
+	const float *gradImage; double *gAlb, *gEmit; int nextRow; pthread_mutex_t lock;
+} AdjointJob;
+
+static void adjointRows(AdjointJob *j, double *gAlb, double *gEmit) {
+	for (;;) {
+		pthread_mutex_lock(&j->lock);
+		int y = j->nextRow++;
+		pthread_mutex_unlock(&j->lock);
+		if (y >= j->height) break;
+		for (int x = 0; x < j->width; x++) {
+			uint32_t pixel = (uint32_t) y * (uint32_t) j->width + (uint32_t) x;
 			double delta[3];
-			for (int c = 0; c < 3; c++) delta[c] = (double) gradImage[3 * pixel + c] / (double) spp;
-			for (int so = 0; so < spp; so++) {
+			for (int c = 0; c < 3; c++) delta[c] = (double) j->gradImage[3 * pixel + c] / (double) j->spp;
+			for (int so = 0; so < j->spp; so++) {
 				uint32_t rng;
 				v3 pos, dir;
-				primaryRay(cam, width, height, x, y, frameIndex, frameSeed, so, &rng, &pos, &dir);
-				adjointPath(s, pos, dir, maxDepth, rrStart, &rng, delta, gAlb, gEmit);
+				primaryRay(j->cam, j->width, j->height, x, y, j->frameIndex, j->frameSeed, so, &rng, &pos, &dir);
+				adjointPath(j->s, pos, dir, j->maxDepth, j->rrStart, &rng, delta, gAlb, gEmit);
 			}
 		}
+	}
+}
+
+typedef struct { AdjointJob *job; double *gAlb, *gEmit; } AdjointWorker;
+
+static void *adjointWorker(void *arg) {
+	AdjointWorker *w = (AdjointWorker *) arg;
+	adjointRows(w->job, w->gAlb, w->gEmit);
+	return NULL;
+}
+
+static int primitiveTotal(const OrScene *s) {
+	return s->sphereCount + s->boxCount + s->affineSphereCount + s->affineBoxCount + s->planeCount;
+}
+
+int or_adjoint(const OrScene *s, const float *cam, int width, int height, int maxDepth, int rrStart, int frameIndex,
+		uint64_t frameSeed, int spp, const float *gradImage, double *gAlb, double *gEmit, int threads) {
+	if (!s || !cam || !gradImage || !gAlb || !gEmit || spp < 1 || maxDepth > 32) return -1;
+	AdjointJob j = {s, cam, width, height, maxDepth, rrStart, frameIndex, spp, frameSeed, gradImage, gAlb, gEmit, 0};
+	pthread_mutex_init(&j.lock, NULL);
+	if (threads < 1) threads = 1;
+	if (threads > 64) threads = 64;
+	size_t n = 3 * (size_t) primitiveTotal(s);
+	pthread_t tid[64];
+	AdjointWorker w[64];
+	for (int i = 1; i < threads; i++) {
+		w[i].job = &j;
+		w[i].gAlb = (double *) calloc(n ? n : 1, sizeof(double));
+		w[i].gEmit = (double *) calloc(n ? n : 1, sizeof(double));
+		pthread_create(&tid[i], NULL, adjointWorker, &w[i]);
+	}
+	adjointRows(&j, gAlb, gEmit);
+	for (int i = 1; i < threads; i++) {
+		pthread_join(tid[i], NULL);
+		for (size_t k = 0; k < n; k++) { gAlb[k] += w[i].gAlb[k]; gEmit[k] += w[i].gEmit[k]; }
+		free(w[i].gAlb);
+		free(w[i].gEmit);
+	}
+	pthread_mutex_destroy(&j.lock);
 	return 0;
 }
 
@@ -932,6 +980,82 @@ int or_render_fixed_paths64(const OrScene *s, const float *cam, int width, int h
 			}
 			for (int c = 0; c < 3; c++) out[3 * pixel + c] = acc[c] / spp;
 		}
+	return 0;
+}
+
+/* The vertices of the frame's paths as tracePath follows them without
+ * roulette (FP32 geometry, the scene's own materials for every sampling
+ * decision), for re-evaluating the frame's estimator in double under other
+ * material parameters: per path (index pixel * spp + sample) the material
+ * index, lobe (0 diffuse, 1 reflective, 2 refractive) and lobe probability of
+ * each hit vertex, the number of vertices, and how the path ended (1 = its
+ * last ray missed and gathered the background, 2 = absorbed at its last
+ * vertex (all lobe weights zero), 0 = ran out of depth).  Arrays hold
+ * maxDepth vertices per path.  Rows are split over `threads` threads. */
+typedef struct {
+	const OrScene *s; const float *cam; int width, height, maxDepth, frameIndex, spp; uint64_t frameSeed;
+	int32_t *mat; uint8_t *lobe; float *prob; uint8_t *count; uint8_t *end;
+	int nextRow; pthread_mutex_t lock;
+} RecordJob;
+
+static void recordPath(const OrScene *s, v3 o, v3 d, int maxDepth, uint32_t *rng, int32_t *mat, uint8_t *lobe,
+		float *prob, uint8_t *count, uint8_t *end) {
+	int n = 0;
+	*end = 0;
+	for (int depth = 0; depth < maxDepth; ++depth) {
+		Hit h;
+		if (!traceClosest(s, o, d, &h, NULL)) { *end = 1; break; }
+		mat[n] = h.matIndex;
+		v3 nrm = h.normal, in = vneg(d);
+		float wD = luminance(h.m.diffuse), wR = luminance(h.m.reflective), wT = luminance(h.m.refractive);
+		float sum = wD + wR + wT;
+		if (sum <= 0.0f) { lobe[n] = 0; prob[n] = 0.0f; n++; *end = 2; break; }
+		float xi = rand01(rng) * sum;
+		v3 nd;
+		if (xi < wD) { prob[n] = wD / sum; nd = vadd(sampleUnitSphere(rng), nrm); lobe[n] = 0; }
+		else if (xi < wD + wR) { prob[n] = wR / sum; nd = reflect_dir(nrm, in); lobe[n] = 1; }
+		else { prob[n] = wT / sum; v3 rd; refract_dir(in, nrm, h.m.ior, &rd); nd = rd; lobe[n] = 2; }
+		n++;
+		o = vadd(h.pos, vmuls(nd, EPSILON));
+		d = nd;
+	}
+	*count = (uint8_t) n;
+}
+
+static void *recordWorker(void *arg) {
+	RecordJob *j = (RecordJob *) arg;
+	for (;;) {
+		pthread_mutex_lock(&j->lock);
+		int y = j->nextRow++;
+		pthread_mutex_unlock(&j->lock);
+		if (y >= j->height) break;
+		for (int x = 0; x < j->width; x++)
+			for (int so = 0; so < j->spp; so++) {
+				size_t path = ((size_t) y * j->width + x) * j->spp + so;
+				size_t v = path * j->maxDepth;
+				uint32_t rng;
+				v3 o, d;
+				primaryRay(j->cam, j->width, j->height, x, y, j->frameIndex, j->frameSeed, so, &rng, &o, &d);
+				recordPath(j->s, o, d, j->maxDepth, &rng, j->mat + v, j->lobe + v, j->prob + v, j->count + path,
+				           j->end + path);
+			}
+	}
+	return NULL;
+}
+
+int or_record_paths(const OrScene *s, const float *cam, int width, int height, int maxDepth, int frameIndex,
+		uint64_t frameSeed, int spp, int threads, int32_t *mat, uint8_t *lobe, float *prob, uint8_t *count,
+		uint8_t *end) {
+	if (!s || !cam || !mat || !lobe || !prob || !count || !end || spp < 1 || maxDepth < 0 || maxDepth > 32) return -1;
+	RecordJob j = {s, cam, width, height, maxDepth, frameIndex, spp, frameSeed, mat, lobe, prob, count, end, 0};
+	pthread_mutex_init(&j.lock, NULL);
+	if (threads < 1) threads = 1;
+	if (threads > 256) threads = 256;
+	pthread_t tid[256];
+	for (int i = 1; i < threads; i++) pthread_create(&tid[i], NULL, recordWorker, &j);
+	recordWorker(&j);
+	for (int i = 1; i < threads; i++) pthread_join(tid[i], NULL);
+	pthread_mutex_destroy(&j.lock);
 	return 0;
 }
 

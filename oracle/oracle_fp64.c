@@ -298,6 +298,7 @@ static d3 radiance64(const OrScene *s, d3 p, d3 d, int maxDepth, int rrStart, ui
 typedef struct {
 	const OrScene *s; const float *cam; int width, height, maxDepth, rrStart, spp;
 	uint64_t seed; double *mean, *var; int nextRow; pthread_mutex_t lock;
+	int x0, y0, cw, ch;     /* the window rendered; mean/var are cw*ch*3, row-major over the window */
 } Job64;
 
 static void *worker64(void *arg) {
@@ -309,8 +310,11 @@ static void *worker64(void *arg) {
 		pthread_mutex_lock(&j->lock);
 		int y = j->nextRow++;
 		pthread_mutex_unlock(&j->lock);
-		if (y >= j->height) break;
-		for (int x = 0; x < j->width; x++) {
+		if (y >= j->ch) break;
+		const int wy = y;
+		y += j->y0;
+		for (int wx = 0; wx < j->cw; wx++) {
+			const int x = j->x0 + wx;
 			uint64_t rng = j->seed ^ (0xD1B54A32D192ED03ull * (uint64_t) ((uint64_t) y * j->width + x + 1));
 			d3 s1 = D(0, 0, 0), s2 = D(0, 0, 0);
 			for (int k = 0; k < j->spp; k++) {
@@ -321,7 +325,7 @@ static void *worker64(void *arg) {
 				s1 = add(s1, c);
 				s2 = add(s2, hmul(c, c));
 			}
-			size_t idx = 3 * ((size_t) y * j->width + x);
+			size_t idx = 3 * ((size_t) wy * j->cw + wx);
 			double n = j->spp;
 			d3 m = mul(s1, 1.0 / n);
 			j->mean[idx] = m.x; j->mean[idx + 1] = m.y; j->mean[idx + 2] = m.z;
@@ -338,10 +342,12 @@ static void *worker64(void *arg) {
 
 /* Renders spp samples per pixel; mean and (optional) per-pixel sample
  * variance of the single-sample estimator, both W*H*3 doubles. */
-int or64_render(const OrScene *s, const float *cam, int width, int height, int maxDepth, int rrStart, int spp,
-		uint64_t seed, double *mean, double *var, int threads) {
+int or64_render_window(const OrScene *s, const float *cam, int width, int height, int x0, int y0, int cw, int ch,
+		int maxDepth, int rrStart, int spp, uint64_t seed, double *mean, double *var, int threads) {
 	if (!s || !cam || !mean || width < 1 || height < 1 || spp < 1) return -1;
+	if (x0 < 0 || y0 < 0 || cw < 1 || ch < 1 || x0 + cw > width || y0 + ch > height) return -1;
 	Job64 j = {s, cam, width, height, maxDepth, rrStart, spp, seed, mean, var, 0};
+	j.x0 = x0; j.y0 = y0; j.cw = cw; j.ch = ch;
 	pthread_mutex_init(&j.lock, NULL);
 	if (threads < 1) threads = 1;
 	if (threads > 256) threads = 256;
@@ -351,6 +357,12 @@ int or64_render(const OrScene *s, const float *cam, int width, int height, int m
 	for (int i = 1; i < threads; i++) pthread_join(tid[i], NULL);
 	pthread_mutex_destroy(&j.lock);
 	return 0;
+}
+
+int or64_render(const OrScene *s, const float *cam, int width, int height, int maxDepth, int rrStart, int spp,
+		uint64_t seed, double *mean, double *var, int threads) {
+	return or64_render_window(s, cam, width, height, 0, 0, width, height, maxDepth, rrStart, spp, seed, mean, var,
+	                          threads);
 }
 
 /* Closest hit in FP64 for one ray (distance, kind, index, normal). */

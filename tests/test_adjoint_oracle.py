@@ -65,47 +65,52 @@ def test_oracle_adjoint_matches_central_differences_room(rr):
     assert checked >= 21
 
 
-def test_adjoint_central_differences_diff_field_fixed_paths():
-    """Heterogeneous 10k field, 4 bounces: the adjoint of the frame's paths against central differences
-    (h = 1e-3) of the same FP32 paths re-evaluated in double, on 64 parameters (56 strongest albedo
-    channels and 8 emitter radiances), within 1e-3 relative."""
+def test_recorded_paths_reproduce_the_fixed_path_renderer():
+    """The recorded vertices of a frame (or_record_paths), re-evaluated in numpy, give the FP64 fixed-path
+    frame of or_render_fixed_paths64 exactly, for the scene's materials and for perturbed ones."""
     w = configs.diff(10_000)
     wire, cam = w.scene.wire(), w.camera.packed()
-    Wd, Hd, spp = 40, 40, 2
     osc = O.Scene(wire)
     mats = wire["materials"].astype(np.float64)
+    rec = O.RecordedPaths(osc, cam, 40, 30, w.max_depth, 0, SEED, 3)
+    pert = mats * np.random.default_rng(4).uniform(0.8, 1.2, mats.shape)
+    for m in (mats, pert):
+        ref = O.render_fixed_paths64(osc, cam, 40, 30, w.max_depth, 0, SEED, 3, m)
+        np.testing.assert_array_equal(rec.image(m), ref)
+    # the room: boxes, spheres, an affine box, absorbing black surfaces and an emitter with zero albedo
+    r = configs.room()
+    osc = O.Scene(r.scene.wire())
+    rm = r.scene.wire()["materials"].astype(np.float64)
+    rec = O.RecordedPaths(osc, r.camera.packed(), 24, 24, r.max_depth, 0, SEED, 4)
+    assert (rec.end == 2).any() and (rec.end == 0).any()
+    np.testing.assert_array_equal(rec.image(rm), O.render_fixed_paths64(osc, r.camera.packed(), 24, 24, r.max_depth,
+                                                                        0, SEED, 4, rm))
+
+
+def test_adjoint_central_differences_diff_field_fixed_paths():
+    """Heterogeneous 10k field, 4 bounces: the oracle adjoint of the frame's paths against central differences
+    (h = 1e-3) of the same FP32 paths re-evaluated in double, on 64 parameters drawn at random among those with
+    a non-zero gradient (56 albedo channels and 8 emitter radiances), all within 1e-3 relative."""
+    from fd_paths import PathFD, draw_parameters, per_material
+    w = configs.diff(10_000)
+    wire, cam = w.scene.wire(), w.camera.packed()
+    osc = O.Scene(wire)
+    mats = wire["materials"].astype(np.float64)
+    rec = O.RecordedPaths(osc, cam, 96, 96, w.max_depth, 0, SEED, 4)
     rng = np.random.default_rng(9)
     tm = mats.copy()
     tm[:, 0:3] *= rng.uniform(0.8, 1.2, (len(mats), 3))
     tm[:, 9:12] *= rng.uniform(0.8, 1.2, (len(mats), 1))
-    img64 = lambda m: O.render_fixed_paths64(osc, cam, Wd, Hd, w.max_depth, 0, SEED, spp, m)  # noqa: E731
-    target = img64(tm)
-    base = img64(mats)
-    loss = lambda img: float(np.mean((img - target) ** 2))  # noqa: E731
-    gimg = (2.0 * (base - target) / base.size).astype(np.float32)
+    fd = PathFD(rec, mats, rec.image(tm))
     prim_mat = primitive_material_indices(w.scene)
-    ga, ge = O.adjoint(osc, cam, Wd, Hd, w.max_depth, 0, SEED, spp, gimg, len(prim_mat))
-    gam, gem = _per_material(ga, prim_mat, len(mats)), _per_material(ge, prim_mat, len(mats))
-    picks = [("albedo", int(i // 3), int(i % 3)) for i in np.argsort(-np.abs(gam).reshape(-1))[:56]]
-    emitters = np.where(mats[:, 9] > 0)[0]
-    picks += [("emission", int(m), -1) for m in emitters[np.argsort(-np.abs(gem[emitters].sum(1)))][:8]]
-    h = 1e-3
-    worst = 0.0
+    ga, ge = O.adjoint(osc, cam, 96, 96, w.max_depth, 0, SEED, 4, fd.grad_image(), len(prim_mat))
+    gam, gem = per_material(ga, prim_mat, len(mats)), per_material(ge, prim_mat, len(mats))
+    picks = draw_parameters(np.random.default_rng(64), gam, gem, np.flatnonzero(mats[:, 9] > 0))
+    assert len(picks) == 64 and sum(k == "emission" for k, _, _ in picks) == 8
     for kind, m, c in picks:
-        plus, minus = mats.copy(), mats.copy()
-        if kind == "albedo":
-            plus[m, c] += h
-            minus[m, c] -= h
-            analytic = gam[m, c]
-        else:
-            plus[m, 9:12] += h
-            minus[m, 9:12] -= h
-            analytic = gem[m].sum()
-        fd = (loss(img64(plus)) - loss(img64(minus))) / (2 * h)
-        rel = abs(fd - analytic) / max(abs(fd), 1e-300)
-        worst = max(worst, rel)
-        assert rel <= 1e-3, (kind, m, c, fd, analytic)
-    assert worst < 1e-3
+        f = fd.central_difference(kind, m, c, 1e-3)
+        g = gam[m, c] if kind == "albedo" else gem[m].sum()
+        assert abs(f - g) <= 1e-3 * abs(f), (kind, m, c, f, g)
 
 
 def test_emission_gradient_is_linear_response():

@@ -1,6 +1,6 @@
 """GPU: the path-replay adjoint against the oracle adjoint (same paths, FP64
-sums) and against central finite differences of the GPU forward pass with
-common random numbers (h = 1e-3)."""
+sums) and against central finite differences (h = 1e-3) with common random
+numbers on the differentiable configuration."""
 from __future__ import annotations
 
 import numpy as np
@@ -46,58 +46,47 @@ def test_gpu_adjoint_matches_oracle(require_gpu, case):
     r.close()
 
 
-def _fd_rounding_bound(img_p, img_m, delta, spp, span):
-    """Largest loss-difference error FP32 rounding of the two images can cause, over the step span:
-    each changed pixel is a sum of spp path radiances, each rounded in FP32."""
-    changed = img_p != img_m
-    ulp = np.spacing(np.maximum(np.abs(img_p), np.abs(img_m)).astype(np.float32)).astype(np.float64)
-    return float(np.sum(np.abs(delta[changed]) * 4.0 * spp * ulp[changed])) / span
-
-
-def test_gpu_adjoint_central_differences_diff_field(require_gpu):
-    """Heterogeneous field at 10k primitives (4 bounces, no roulette) at 128x128, 8 spp: the 56 strongest
-    albedo channels and 8 emitter radiances, central differences h = 1e-3 of the GPU forward pass.
-    The agreement bar is 1e-3 relative plus the FP32 rounding bound of the two images (a bright pixel,
-    e.g. one with an emitter sample, resolves a 1e-3 step of a dim path only to ~1e-3); most parameters
-    meet 1e-3 outright."""
+def test_gpu_adjoint_central_differences_diff_config(require_gpu):
+    """The differentiable configuration itself: the 10k heterogeneous field at 512x512, 32 spp, 4 bounces, loss
+    the pixel L2 to the frame under seed-perturbed albedos and emissions (each value scaled by U[0.8, 1.2]).
+    The GPU path-replay gradients of 64 parameters drawn at random (seeded) among those with a non-zero
+    gradient, 56 albedo channels and 8 emitter radiances, are within 1e-3 of central differences (h = 1e-3,
+    common random numbers: the frame's own paths, re-evaluated in double as an exact polynomial in the
+    parameters), and every gradient of weight equals the FP64-summed oracle adjoint of the same paths to 1e-4."""
+    from fd_paths import PathFD, draw_parameters, per_material
     w = configs.diff(10_000)
-    W, H, SPP = 128, 128, 8
-    r = PathReplayRenderer(w.scene, w.camera, W, H, SPP, w.max_depth, frame_seed=0xD1FF)
-    target = _target(r, 9)
-    base = r.base_materials
-    img = r.forward(base)
-    loss0, gimg = l2_loss(img, target)
+    seed = 0xD1FF
+    osc = O.Scene(w.scene.wire())
+    mats = w.scene.material_data().astype(np.float64)
+    print("recording the diff-10k frame's paths (512x512, 32 spp)", flush=True)
+    rec = O.RecordedPaths(osc, w.camera.packed(), w.width, w.height, w.max_depth, 0, seed, w.spp)
+    rng = np.random.default_rng(9)
+    tm = mats.copy()
+    tm[:, 0:3] *= rng.uniform(0.8, 1.2, (len(mats), 3))
+    tm[:, 9:12] *= rng.uniform(0.8, 1.2, (len(mats), 1))
+    fd = PathFD(rec, mats, rec.image(tm))
+    gimg = fd.grad_image()
+    r = PathReplayRenderer(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, frame_seed=seed)
+    print("GPU backward", flush=True)
     ga, ge = r.backward(gimg)
-    gam, gem = r.per_material(ga), r.per_material(ge)
-    # 64 parameters with a visible effect on the image: the 56 largest albedo gradients and 8 emitters
-    order = np.argsort(-np.abs(gam).reshape(-1))[:56]
-    picks = [("albedo", int(i // 3), int(i % 3)) for i in order]
-    emitters = np.where(base[:, 9] > 0)[0]
-    em_order = emitters[np.argsort(-np.abs(gem[emitters].sum(1)))][:8]
-    picks += [("emission", int(m), -1) for m in em_order]
-    h = 1e-3
-    bad, strict = [], 0
-    delta = gimg.astype(np.float64)
-    for kind, m, c in picks:
-        plus, minus = base.copy(), base.copy()
-        if kind == "albedo":
-            plus[m, c] += np.float32(h); minus[m, c] -= np.float32(h)
-            analytic = gam[m, c]
-            span = np.float64(plus[m, c]) - np.float64(minus[m, c])
-        else:   # grey radiance: all three channels scaled together
-            plus[m, 9:12] += np.float32(h); minus[m, 9:12] -= np.float32(h)
-            analytic = gem[m].sum()
-            span = np.float64(plus[m, 9]) - np.float64(minus[m, 9])
-        ip, im = r.forward(plus), r.forward(minus)
-        fd = (l2_loss(ip, target)[0] - l2_loss(im, target)[0]) / span
-        err = abs(fd - analytic)
-        strict += err <= 1e-3 * abs(fd)
-        if err > 1e-3 * abs(fd) + _fd_rounding_bound(ip, im, delta, SPP, span):
-            bad.append((kind, m, c, fd, analytic))
-    r.forward(base)
+    prim_mat, n_prim = r.prim_material, r.primitive_count
     r.close()
+    oa, oe = O.adjoint(osc, w.camera.packed(), w.width, w.height, w.max_depth, 0, seed, w.spp, gimg, n_prim)
+    for got, ref in ((ga, oa), (ge, oe)):
+        big = np.abs(ref) > 1e-3 * np.abs(ref).max()
+        assert big.sum() > 1000
+        np.testing.assert_allclose(got[big], ref[big], rtol=1e-4)
+    gam, gem = per_material(ga, prim_mat, len(mats)), per_material(ge, prim_mat, len(mats))
+    picks = draw_parameters(np.random.default_rng(64), gam, gem, np.flatnonzero(mats[:, 9] > 0))
+    assert len(picks) == 64
+    print("central differences of 64 parameters", flush=True)
+    bad = []
+    for kind, m, c in picks:
+        f = fd.central_difference(kind, m, c, 1e-3)
+        g = gam[m, c] if kind == "albedo" else gem[m].sum()
+        if not abs(f - g) <= 1e-3 * abs(f):
+            bad.append((kind, m, c, f, g))
     assert not bad, bad
-    assert strict >= len(picks) // 2, strict
 
 
 def test_gpu_adjoint_matches_oracle_per_parameter_diff_field(require_gpu):
