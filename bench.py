@@ -348,13 +348,13 @@ def bench_batch(args) -> dict:
         "gi_test_context": gi,
         "roofline": roofline_entry("wfTrace", trace_s / launches, launches, trace_s * 1e3,
                                    algorithmic_bytes(st_m) / launches, load_traffic("batch-64"),
-                                   load_counters("batch-64")),
+                                   load_counters("batch-64"), "batch-64"),
         "counters_per_step": {k: st_m[k] for k in mrad.STAT_NAMES},
     }, ws
 
 
 def roofline_entry(kernel: str, launch_s: float, launches: int, kernel_ms_per_step: float,
-                   logical_bytes_per_launch: float, traffic, counters) -> dict:
+                   logical_bytes_per_launch: float, traffic, counters, profile: str) -> dict:
     """The roofline object of the dominant kernel.  The roof that binds is VALU issue: `achieved` is the
     kernel's wave-level VALU instructions per launch (SQ_INSTS_VALU, rocprofv3 counters of the same render)
     over its average launch time measured here, in G instructions/s, against VALU_PEAK_GINST.
@@ -368,7 +368,7 @@ def roofline_entry(kernel: str, launch_s: float, launches: int, kernel_ms_per_st
     if counters and counters.get("SQ_INSTS_VALU"):
         ach = counters["SQ_INSTS_VALU"] / launch_s / 1e9
         r.update(achieved=ach, frac=ach / VALU_PEAK_GINST, valu_insts_per_launch=counters["SQ_INSTS_VALU"],
-                 counters="profiles/counters_batch-64.json")
+                 counters=f"profiles/counters_{profile}.json")
         if counters.get("SQ_ACTIVE_INST_VALU") and counters.get("SQ_THREAD_CYCLES_VALU"):
             r["lane_efficiency"] = counters["SQ_THREAD_CYCLES_VALU"] / (64.0 * counters["SQ_ACTIVE_INST_VALU"])
     if traffic is not None and launch_s > 0:
@@ -473,7 +473,7 @@ def bench_single(args, w: configs.Workload) -> dict:
         "forms": forms,
         "roofline": roofline_entry("renderKernel", kernel_s / args.steps, 1, kernel_s / args.steps * 1e3,
                                    achieved * 1e9 * kernel_s / args.steps, load_traffic(w.name),
-                                   load_counters(w.name)),
+                                   load_counters(w.name), w.name),
         "counters_per_step": {k: totals[k] / args.steps for k in mrad.STAT_NAMES},
     }, [w]
 
@@ -646,7 +646,7 @@ def bench_hero(args) -> dict:
         "scales": rows,
         "roofline": roofline_entry("renderKernel", line["best_kernel_ms"] / 1e3, 1, line["best_kernel_ms"],
                                    line["roofline_gbs"] * line["best_kernel_ms"] * 1e6, load_traffic("hero-1m"),
-                                   load_counters("hero-1m")),
+                                   load_counters("hero-1m"), "hero-1m"),
         "counters_per_step": {k: line["counters"][k] for k in mrad.STAT_NAMES},
     }, [configs.Workload("hero-1m", line["scene"], cam, W.WIDTH, W.HEIGHT, 1, W.MAXIMUM_PATH_DEPTH)]
 

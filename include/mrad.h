@@ -296,6 +296,12 @@ int mrad_cost_create(int device, const float *workload, int records, int profile
  * 6 interior traversal, 7 plane; elapsed_ms from HIP events around it. */
 int mrad_cost_time(mrad_cost *cost, int operation, int profile, int iteration_offset, int iterations, int copies,
                    int blocks, int threads, float *elapsed_ms);
+/* The accumulators the last mrad_cost_time launch wrote, one float per
+ * thread in launch order (n <= blocks * threads of that launch): every
+ * operation's results folded as the reference's primitiveCostLoop folds its
+ * own (PrimitiveCostBenchmark.java:820-841), so each operation can be checked
+ * against its intersector. */
+int mrad_cost_output(mrad_cost *cost, float *out, int n);
 void mrad_cost_destroy(mrad_cost *cost);
 
 #ifdef __cplusplus

@@ -250,4 +250,16 @@ extern "C" int mrad_cost_time(mrad_cost *c, int operation, int profile, int iter
 	return MRAD_OK;
 }
 
+// The per-thread accumulators of the last mrad_cost_time launch (blocks x
+// threads floats, thread order).
+extern "C" int mrad_cost_output(mrad_cost *c, float *out, int n) {
+	clear_error();
+	if (!c || !out || n <= 0) return set_error(MRAD_ERR_INVALID_ARGUMENT, "calibration output: handle, buffer, count");
+	if ((size_t) n * sizeof(float) > c->out.cap || !c->out.p)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "calibration output: more values than the last launch wrote");
+	MRAD_HIP(hipSetDevice(c->device));
+	MRAD_HIP(hipMemcpy(out, c->out.p, (size_t) n * sizeof(float), hipMemcpyDeviceToHost));
+	return MRAD_OK;
+}
+
 extern "C" void mrad_cost_destroy(mrad_cost *c) { cost_free(c); }

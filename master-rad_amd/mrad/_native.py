@@ -135,6 +135,7 @@ SIGNATURES.update({
     "mrad_update_materials": (C.c_int, [C.c_void_p, fp, C.c_int]),
     "mrad_cost_create": (C.c_int, [C.c_int, fp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p)]),
     "mrad_cost_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp]),
+    "mrad_cost_output": (C.c_int, [C.c_void_p, fp, C.c_int]),
     "mrad_cost_destroy": (None, [C.c_void_p]),
 })
 

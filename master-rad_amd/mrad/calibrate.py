@@ -185,6 +185,12 @@ class _Timer:
                                      C.byref(ms)))
         return float(ms.value)
 
+    def output(self, n: int) -> np.ndarray:
+        """The per-thread accumulators of the last launch (mrad_cost_output)."""
+        out = np.zeros(n, np.float32)
+        N.check(N.lib.mrad_cost_output(self._h, N.ptr(out, N.fp), n))
+        return out
+
     def close(self):
         if self._h is not None and self._h.value:
             N.lib.mrad_cost_destroy(self._h)

@@ -7,6 +7,7 @@
     python -m mrad.profile_run hero       # Layered Hero 1M weighted, 1080p, 1 spp, depth 12: 5 megakernel
                                           # frames with leaf batching (16 lanes)
     python -m mrad.profile_run random     # F4-10k layout 0 weighted, 1080p, 1 spp, depth 12: 5 megakernel frames
+    python -m mrad.profile_run clustered  # clustered-1M, 1080p, 16 spp, 6 bounces: megakernel frame + wavefront
 
 Run from master-rad_amd/ (or with it on sys.path).  Each prints one line per
 step so a profiled run shows progress.
@@ -65,6 +66,25 @@ def _copy(nbytes: int = 1 << 30):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     print(f"copy {nbytes} bytes in {dt * 1e3:.2f} ms ({nbytes * 2 / dt / 1e9:.0f} GB/s read+write)", flush=True)
+
+
+def _clustered():
+    """The clustered configuration (1M primitives in 64 Gaussian clusters, half affine, a ground box, 0.5 %
+    emitters, BVH leaf 4; 1920x1080, 16 spp, 6 bounces): one megakernel frame, then the same frame in the
+    wavefront form."""
+    import mrad
+    from mrad import configs
+    w = configs.clustered()
+    w.scene.bvh_stats()
+    print("clustered-1M built", flush=True)
+    with mrad.GpuRayTracer(w.width, w.height, w.spp, True) as t:
+        t.render_sample(None, w.scene, w.camera, w.max_depth, 0, 7)
+        print(f"megakernel frame {t.last_frame_stats()['kernel_nanos'] / 1e6:.1f} ms", flush=True)
+    b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 25))
+    b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, 0, 7)
+    st = b.render()
+    print(f"wavefront render {st['kernel_nanos'] / 1e6:.1f} ms, {st['steps']} steps", flush=True)
+    b.close()
 
 
 def _adjoint():
@@ -152,6 +172,8 @@ def main(argv):
         _hero()
     elif what == "random":
         _random()
+    elif what == "clustered":
+        _clustered()
     else:
         raise SystemExit(f"unknown workload {what}")
 

@@ -67,6 +67,8 @@ def _lib() -> C.CDLL:
     lib.or_adjoint.restype = C.c_int
     lib.or_adjoint.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                C.c_int, fp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    lib.or_cost_accumulator.restype = C.c_float
+    lib.or_cost_accumulator.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     lib.or_record_paths.restype = C.c_int
     lib.or_record_paths.argtypes = [C.POINTER(OrScene), fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,
                                     C.c_int, ip, C.POINTER(C.c_uint8), fp, C.POINTER(C.c_uint8),
@@ -309,3 +311,12 @@ def closest64(scene: Scene, ray6):
     n = (C.c_double * 3)()
     f = lib.or64_closest(C.byref(scene.c), r.ctypes.data_as(fp), C.byref(t), C.byref(kind), C.byref(index), n)
     return bool(f), t.value, kind.value, index.value, np.array(n[:])
+
+
+def cost_accumulators(workload: np.ndarray, records: int, profiles: int, op: int, profile: int, offset: int,
+                      iterations: int, copies: int, threads: int) -> np.ndarray:
+    """or_cost_accumulator of threads 0..threads-1: the calibration kernel's per-thread outputs."""
+    w = np.ascontiguousarray(workload, np.float32)
+    p = w.ctypes.data_as(fp)
+    return np.array([lib.or_cost_accumulator(p, records, profiles, op, profile, offset, iterations, copies, t)
+                     for t in range(threads)], np.float32)

@@ -1059,6 +1059,74 @@ int or_record_paths(const OrScene *s, const float *cam, int width, int height, i
 	return 0;
 }
 
+/* The per-thread accumulator of the library's calibration kernel (costKernel
+ * over its traversal-time operations), restated with this oracle's
+ * intersectors: thread tid runs `iterations` iterations from iteration
+ * `offset`; each picks a ray of `profile` and `copies` records by hash
+ * (PrimitiveCostBenchmark.java:820-841) and folds one operation's result into
+ * the float accumulator.  Operation 0 setup, 1 sphere, 2 box, 3 affine sphere,
+ * 4 affine box (the reference's routines, GpuKernelSources.java:338-706, with
+ * the running best as the accept bound), 5 one node entry distance, 6 a node
+ * pair's two entry distances ordered near first (bvhEntryDistance, :736-770;
+ * ordering as traceBvh :915-922), 7 plane.  The workload is the calibration
+ * layout: profiles x records rays, then spheres, boxes, affine records,
+ * planes, node AABBs, node pairs, materials. */
+typedef struct { float t; uint32_t rank; int kind, index; } CostBest;
+
+static float costConsume(const CostBest *b) {
+	return b->t * 0.000001f + (float) b->kind * 0.000073f + (float) b->rank * 0.0000001f + (float) b->index * 0.0000003f;
+}
+
+float or_cost_accumulator(const float *w, int records, int profiles, int op, int profile, int offset, int iterations,
+		int copies, int tid) {
+	const size_t n = (size_t) records;
+	const float *rays = w, *sph = rays + (size_t) profiles * n * 6, *box = sph + n * 5, *aff = box + n * 7;
+	const float *pla = aff + n * 22, *aabb = pla + n * 26, *pair = aabb + n * 6, *mats = pair + n * 12;
+	const uint32_t mask = (uint32_t) records - 1u;
+	float acc = 0.0f;
+	for (int i = 0; i < iterations; i++) {
+		const uint32_t sample = or_hash_u32((uint32_t) (offset + i) * 0x9E3779B9u + (uint32_t) tid);
+		const float *r = rays + ((size_t) profile * n + (sample & mask)) * 6;
+		const v3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+		for (int c = 0; c < copies; c++) {
+			const uint32_t rs = or_hash_u32(sample ^ ((uint32_t) (c + 1) * 0x85EBCA6Bu));
+			const int rec = (int) (rs & mask);
+			CostBest b = {64.0f + fabsf(o.x) + fabsf(o.y) + fabsf(o.z), (uint32_t) rec, 0, rec};
+			Hit h;
+			memset(&h, 0, sizeof h);
+			h.t = b.t;
+			int found = 0, kind = op;
+			switch (op) {
+				case 1: found = traceSphere(sph + 5 * (size_t) rec, 1, mats, o, d, 0, &h); break;
+				case 2: found = traceBox(box + 7 * (size_t) rec, 1, mats, o, d, 0, &h); break;
+				case 3: found = traceAffineSphere(aff + 22 * (size_t) rec, 1, mats, o, d, 0, &h); break;
+				case 4: found = traceAffineBox(aff + 22 * (size_t) rec, 1, mats, o, d, 0, &h); break;
+				case 5: {
+					const float e = or_bvh_entry_distance(aabb, rec, o, d, b.t);
+					if (isfinite(e)) { found = 1; h.t = e; }
+					break;
+				}
+				case 6: {
+					const float *pb = pair + 12 * (size_t) rec;
+					const float le = or_bvh_entry_distance(pb, 0, o, d, b.t), re = or_bvh_entry_distance(pb, 1, o, d, b.t);
+					const int swap = re < le;
+					const int nearN = swap ? 1 : 0, farN = swap ? 0 : 1;
+					const float nearE = swap ? re : le, farE = swap ? le : re;
+					if (isfinite(nearE)) { found = 1; h.t = nearE; b.index = nearN; }
+					else if (isfinite(farE)) { found = 1; h.t = farE; b.index = farN; }
+					break;
+				}
+				case 7: found = tracePlane(pla + 26 * (size_t) rec, 1, mats, o, d, 0, &h); break;
+				default: break;
+			}
+			if (found) { b.t = h.t; b.kind = kind; }
+			acc += o.x * 0.00013f + o.y * 0.00017f + o.z * 0.00019f + d.x * 0.00023f + d.y * 0.00029f +
+			       d.z * 0.00031f + (float) (rec & 7) * 0.0000001f + costConsume(&b);
+		}
+	}
+	return acc;
+}
+
 /* traceReplayKernel body, GpuKernelSources.java:1338-1376, with the per-ray
  * counters it computes kept per ray (perRay: n x 14 u32, may be NULL). */
 int or_trace_replay(const OrScene *s, const float *rays, int n, int *flags, float *dist, int *order,
