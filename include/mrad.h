@@ -133,6 +133,12 @@ typedef struct {
 	int bvh_stack_size;       /* gfxlab.gpu.bvhStackSize, 8..256 (0 = 32) */
 	int samples_per_launch;   /* physical launches per frame = ceil(spp / this); 0 = all */
 	int device;               /* HIP device ordinal */
+	int pixels_per_launch;    /* gfxlab.gpu.renderPixelsPerLaunch (GpuLaunchProvenance.java:44-60): 0 = the
+	                             whole image per launch; else tiles of min(W, n) x max(1, n / that) pixels,
+	                             1..262144 */
+	int64_t max_physical_kernel_nanos;  /* 0 = unchecked; else a frame with a physical launch at or above
+	                                       this fails with MRAD_ERR_ILLEGAL_STATE (the reference's 250 ms
+	                                       watchdog guard, GpuRayTracer.java:572-580) */
 } mrad_tracer_config;
 
 /* GpuRayTracer.FrameStats (GpuRayTracer.java:79-99). */
@@ -168,9 +174,17 @@ int mrad_set_leaf_batch(mrad_tracer *tracer, int lanes);
 /* GpuRayTracer.renderSample(rgb, scene, camera, maxDepth, frameIndex,
  * frameSeed) (GpuRayTracer.java:317-432).  camera = position, forward,
  * right, up (GpuCamera.java:8-22).  rgb_out (W*H*3 floats) may be NULL to
- * leave the frame on the device (read it later with mrad_copy_frame). */
+ * leave the frame on the device (read it later with mrad_copy_frame).  The
+ * frame runs as (pixel tiles) x (sample ranges) physical launches; a plan of
+ * more than 8192 launches is an invalid argument, checked before the device
+ * (renderLaunchCount, GpuRayTracer.java:550-564). */
 int mrad_render_sample(mrad_tracer *tracer, float *rgb_out, const float camera[12], int max_depth,
                        int frame_index, uint64_t frame_seed);
+/* GpuRayTracer.renderLaunchCount (GpuRayTracer.java:556-564): the number of
+ * physical launches mrad_render_sample would make for this tracer's
+ * configuration (pixel tiles x sample ranges), or MRAD_ERR_INVALID_ARGUMENT
+ * when it exceeds 8192.  Needs no device. */
+int mrad_render_launch_count(const mrad_tracer *tracer, int *launches);
 /* The device-to-host copy of renderSample (GpuRayTracer.java:397-399). */
 int mrad_copy_frame(mrad_tracer *tracer, float *rgb_out);
 /* GpuRayTracer.traceReplay(scene, rays, n, metrics) (GpuRayTracer.java:593-716)

@@ -156,6 +156,8 @@ struct SceneRange {
 	int rootCount;
 	int roots[4];
 	float rootBounds[4][6];
+	int nodeBase;               // first arena slot of the scene's nodes
+	int topSlots;               // slots from nodeBase holding exactly its top levels (breadth first)
 	int refBase, refCount;
 	int sphBase, sphCount, boxBase, boxCount, asBase, asCount, abBase, abCount;
 	int planeBase, planeCount;
@@ -812,6 +814,20 @@ struct TravStack {
 	__device__ __forceinline__ float entry(int i) const { int n; float e; get(i, n, e); return e; }
 };
 
+// An LDS copy of the first `count` node slots of a scene (from arena slot
+// `base`): the top levels of its BVH, laid out breadth first on upload.  A
+// traversal with kTop > 0 reads those nodes from here instead of HBM.
+typedef float Vec4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) Vec4f LdsFloat4;
+struct TopNodes {
+	const LdsFloat4 *p;
+	int base, count;
+};
+__device__ __forceinline__ float4 ldsLoad4(const LdsFloat4 *q) {
+	const Vec4f v = *q;
+	return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // traceBvh (GpuKernelSources.java:844-961) in three parts: bvhBegin inserts
 // the roots, traceBvh's loop pops and expands stack entries, bvhEnd runs the
 // overflow fallback.  Same stack discipline, same counters.  kCap is the
@@ -876,9 +892,9 @@ __device__ __forceinline__ void orderChildren(int4 nd, float le, float re, int &
 	farE = swap ? le : re;
 }
 
-template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny, int kBatch = 0>
+template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny, int kBatch = 0, int kTop = 0>
 __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
-                                Counters<kMetrics> &c, LdsStack lds) {
+                                Counters<kMetrics> &c, LdsStack lds, TopNodes top = {}) {
 	BvhState<kCap, kShort> b;
 	b.st.lds = lds;
 	bvhBegin<kMetrics, kCap, kShort, kSlab>(R, ray, best, c, b);
@@ -1017,8 +1033,23 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		const float4 *rec = S.nodes + 4 * (size_t) (uint32_t) node;
 #if MRAD_NODE_PREFETCH
 		const float4 a = rec[0], bb = rec[1], cc = rec[2];
-#endif
 		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
+#else
+		float4 a, bb, cc;
+		int4 nd;
+		const uint32_t rel = (uint32_t) node - (uint32_t) top.base;
+		const bool inTop = kTop > 0 && rel < (uint32_t) top.count;
+		if (inTop) {
+			const LdsFloat4 *q = top.p + 4 * rel;
+			const float4 w = ldsLoad4(q + 3);
+			nd = make_int4(__float_as_int(w.x), __float_as_int(w.y), __float_as_int(w.z), __float_as_int(w.w));
+			a = ldsLoad4(q);
+			bb = ldsLoad4(q + 1);
+			cc = ldsLoad4(q + 2);
+		} else {
+			nd = *reinterpret_cast<const int4 *>(rec + 3);
+		}
+#endif
 		if (nd.x < 0) {
 			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
 			ch = cullBound(best.t);
@@ -1026,7 +1057,11 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 		}
 		c.add(10, 1);
 #if !MRAD_NODE_PREFETCH
-		const float4 a = rec[0], bb = rec[1], cc = rec[2];
+		if (!inTop) {
+			a = rec[0];
+			bb = rec[1];
+			cc = rec[2];
+		}
 #endif
 		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
 		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
@@ -1185,9 +1220,9 @@ __device__ inline HitDetail resolveHit(const DevScene &S, f3 o, f3 d, float t, i
 }
 
 // traceClosest (GpuKernelSources.java:963-1020).
-template <bool kMetrics, int kCap, int kShort = kShortStack, int kBatch = 0>
+template <bool kMetrics, int kCap, int kShort = kShortStack, int kBatch = 0, int kTop = 0>
 __device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange &R, f3 o, f3 d, Best &best,
-                                             Counters<kMetrics> &c, LdsStack lds) {
+                                             Counters<kMetrics> &c, LdsStack lds, TopNodes top = {}) {
 	best.t = 1e30f;
 	best.rank = 0xFFFFFFFFu;
 	best.kind = -1;
@@ -1199,14 +1234,14 @@ __device__ __forceinline__ bool traceClosest(const DevScene &S, const SceneRange
 	if (R.useBvh != 0) {
 #if MRAD_SPLIT_SLAB > 1
 		if (ray.fast && d.x != 0.0f && d.y != 0.0f && d.z != 0.0f)
-			traceBvh<kMetrics, kCap, kShort, kSlabFastNz, kBatch>(S, R, ray, best, c, lds);
+			traceBvh<kMetrics, kCap, kShort, kSlabFastNz, kBatch, kTop>(S, R, ray, best, c, lds, top);
 		else
 #endif
-		if (ray.fast) traceBvh<kMetrics, kCap, kShort, kSlabFast, kBatch>(S, R, ray, best, c, lds);
-		else traceBvh<kMetrics, kCap, kShort, kSlabIeee, kBatch>(S, R, ray, best, c, lds);
+		if (ray.fast) traceBvh<kMetrics, kCap, kShort, kSlabFast, kBatch, kTop>(S, R, ray, best, c, lds, top);
+		else traceBvh<kMetrics, kCap, kShort, kSlabIeee, kBatch, kTop>(S, R, ray, best, c, lds, top);
 	}
 #else
-	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort, kSlabAny, kBatch>(S, R, ray, best, c, lds);
+	if (R.useBvh != 0) traceBvh<kMetrics, kCap, kShort, kSlabAny, kBatch, kTop>(S, R, ray, best, c, lds, top);
 #endif
 	else traceLinear<kMetrics>(S, R, ray, best, c);
 	c.add(2, (uint32_t) R.planeCount);
@@ -1308,14 +1343,14 @@ __device__ __forceinline__ bool shadeSegment(const DevScene &S, const SceneRange
 }
 
 // tracePath (GpuKernelSources.java:1061-1165).
-template <bool kMetrics, int kCap, int kBatch = 0>
+template <bool kMetrics, int kCap, int kBatch = 0, int kTop = 0>
 __device__ inline f3 tracePath(const DevScene &S, const SceneRange &R, f3 o, f3 d, int maxDepth, int rrStart,
-                               uint32_t &rng, Counters<kMetrics> &c, LdsStack lds) {
+                               uint32_t &rng, Counters<kMetrics> &c, LdsStack lds, TopNodes top = {}) {
 	f3 T = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
 	for (int depth = 0; depth < maxDepth; ++depth) {
 		c.add(0, 1);
 		Best best;
-		const bool found = traceClosest<kMetrics, kCap, kShortStack, kBatch>(S, R, o, d, best, c, lds);
+		const bool found = traceClosest<kMetrics, kCap, kShortStack, kBatch, kTop>(S, R, o, d, best, c, lds, top);
 		if (!shadeSegment(S, R, found, best, depth, maxDepth, rrStart, o, d, T, L, rng)) break;
 	}
 	return L;

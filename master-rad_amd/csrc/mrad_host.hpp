@@ -20,6 +20,12 @@
 #ifndef MRAD_NODE_PAIRS
 #define MRAD_NODE_PAIRS 1
 #endif
+// Levels below the roots laid out breadth first at the start of a scene's
+// node range (the part kernels may stage in LDS); deeper nodes follow depth
+// first.
+#ifndef MRAD_TOP_LEVELS
+#define MRAD_TOP_LEVELS 5
+#endif
 
 namespace mrad {
 
@@ -119,46 +125,69 @@ struct HostArena {
 
   private:
 	// Arena slot of every node of a scene's BVH, relative to its (even) base;
-	// returns the number of slots.  With MRAD_NODE_PAIRS the two children of
-	// each interior node take slots 2k and 2k + 1, so they share one 128-byte
-	// line: lanes of a wave that part at a node fetch one line, not two.  The
-	// pairs are laid out depth first from each root (a root alone at an even
-	// slot, one pad slot after it).  Storage order is invisible to traversal:
-	// children, bounds and visiting order are the scene's own.  A BVH that is
-	// not a forest of trees over all its nodes (a node reached twice, a child
-	// index out of range, a node no root reaches) keeps the scene's order, and
-	// the validation that follows reports any bad index.
-	static int nodeSlots(const mrad_scene_arrays *a, std::vector<int> &slot) {
+	// returns the number of slots and sets `top` to the number of slots laid
+	// out breadth first.  With MRAD_NODE_PAIRS the two children of each
+	// interior node take slots 2k and 2k + 1, so they share one 128-byte line:
+	// lanes of a wave that part at a node fetch one line, not two.  The roots
+	// come first (each alone at an even slot, one pad slot after it), then the
+	// child pairs of the first MRAD_TOP_LEVELS levels breadth first, so that
+	// the first slots of a scene hold exactly its top levels; below them the
+	// pairs are laid out depth first.  Storage order is invisible to
+	// traversal: children, bounds and visiting order are the scene's own.  A
+	// BVH that is not a forest of trees over all its nodes (a node reached
+	// twice, a child index out of range, a node no root reaches) keeps the
+	// scene's order, and the validation that follows reports any bad index.
+	static int nodeSlots(const mrad_scene_arrays *a, std::vector<int> &slot, int &top) {
 		const int nc = a->node_count;
 		slot.assign((size_t) nc, -1);
+		top = 0;
 		int next = 0;
 		bool ok = MRAD_NODE_PAIRS != 0;
-		std::vector<int> todo;
+		auto pair = [&](int n, std::vector<int> &out) {
+			const int32_t *nd = a->node_data + 4 * (size_t) n;
+			if (nd[0] < 0) return;
+			const int l = nd[0], rr = nd[1];
+			if (l >= nc || rr < 0 || rr >= nc || l == rr || slot[l] >= 0 || slot[rr] >= 0) { ok = false; return; }
+			slot[l] = next;
+			slot[rr] = next + 1;
+			next += 2;
+			out.push_back(l);
+			out.push_back(rr);
+		};
+		std::vector<int> level, below;
 		for (int r = 0; ok && r < a->root_count; r++) {
 			const int ri = a->root_indices[r];
 			if (ri < 0 || ri >= nc) continue;
 			if (slot[ri] >= 0) { ok = false; break; }
 			slot[ri] = next;
 			next += 2;
-			todo.assign(1, ri);
-			while (ok && !todo.empty()) {
-				const int n = todo.back();
-				todo.pop_back();
-				const int32_t *nd = a->node_data + 4 * (size_t) n;
-				if (nd[0] < 0) continue;
-				const int l = nd[0], rr = nd[1];
-				if (l >= nc || rr < 0 || rr >= nc || l == rr || slot[l] >= 0 || slot[rr] >= 0) { ok = false; break; }
-				slot[l] = next;
-				slot[rr] = next + 1;
-				next += 2;
-				todo.push_back(rr);
-				todo.push_back(l);
+			level.push_back(ri);
+		}
+		for (int d = 0; ok && d < MRAD_TOP_LEVELS && !level.empty(); d++) {
+			below.clear();
+			for (int n : level) {
+				pair(n, below);
+				if (!ok) break;
+			}
+			level.swap(below);
+		}
+		top = next;
+		std::vector<int> todo(level.rbegin(), level.rend());
+		while (ok && !todo.empty()) {
+			const int n = todo.back();
+			todo.pop_back();
+			below.clear();
+			pair(n, below);
+			if (!below.empty()) {
+				todo.push_back(below[1]);
+				todo.push_back(below[0]);
 			}
 		}
 		for (int n = 0; ok && n < nc; n++)
 			if (slot[n] < 0) ok = false;
 		if (!ok) {
 			for (int n = 0; n < nc; n++) slot[n] = n;
+			top = 0;
 			return nc;
 		}
 		return next;
@@ -179,7 +208,8 @@ struct HostArena {
 		// node slots: an even base, so that sibling pairs fill 128-byte lines
 		if (nodes.size() / 4 % 2) nodes.resize(nodes.size() + 4, f4(0, 0, 0, 0));
 		std::vector<int> slot;
-		const int slots = nodeSlots(a, slot);
+		int topSlots = 0;
+		const int slots = nodeSlots(a, slot, topSlots);
 		const size_t nodeBase = nodes.size() / 4, refBase = refs.size(), sphBase = spheres.size(),
 		             boxBase = boxes.size() / 2, asBase = affS.size() / 3, abBase = affB.size() / 3,
 		             planeBase = planes.size() / 26, matBase = materials.size() / 4;
@@ -307,6 +337,8 @@ struct HostArena {
 			R.roots[r] = ok ? slot[ri] + (int) nodeBase : -1;
 			for (int k = 0; k < 6; k++) R.rootBounds[r][k] = ok ? a->node_bounds[6 * (size_t) ri + k] : 0.0f;
 		}
+		R.nodeBase = (int) nodeBase;
+		R.topSlots = topSlots;
 		R.refBase = (int) refBase;
 		R.refCount = a->primitive_ref_count;
 		R.sphBase = (int) sphBase; R.sphCount = a->sphere_count;

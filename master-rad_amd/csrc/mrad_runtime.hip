@@ -35,6 +35,10 @@ const char *last_error() { return g_lastError.c_str(); }
 // ---------------------------------------------------------------- kernels
 
 constexpr int kTile = 16;
+// GpuLaunchProvenance.MAX_RENDER_PIXELS_PER_LAUNCH and GpuRayTracer's
+// MAX_RENDER_LAUNCHES_PER_CALL (GpuLaunchProvenance.java:5, GpuRayTracer.java:550-564).
+constexpr int kMaxPixelsPerLaunch = 262144;
+constexpr long long kMaxRenderLaunches = 8192;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
 // XCD i mod 8).  Remapping i so that each XCD receives one contiguous range of
@@ -58,6 +62,7 @@ struct RenderParams {
 	SceneRange R;
 	float cam[12];
 	int width, height, maxDepth, rrStart, frameIndex, s0, s1, total, tilesX;
+	int x0, y0, regionW, regionH;   // the pixel tile of this launch
 	uint64_t frameSeed;
 	float *out;
 	unsigned long long *stats;
@@ -67,6 +72,12 @@ struct RenderParams {
 #ifndef MRAD_RENDER_WAVES_PER_EU
 #define MRAD_RENDER_WAVES_PER_EU 6
 #endif
+// Node slots of the scene's top BVH levels each megakernel block stages in
+// LDS at its start (0: none).
+#ifndef MRAD_RENDER_TOP
+#define MRAD_RENDER_TOP 0
+#endif
+constexpr int kRenderTop = MRAD_RENDER_TOP;
 
 // renderKernel (GpuKernelSources.java:1167-1315).  One thread per pixel runs
 // samples [s0, s1) of the frame and keeps the running sum in registers; the
@@ -79,11 +90,20 @@ void renderKernel(RenderParams P) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int tile = xcdContiguous(blockIdx.x, gridDim.x);
 	const int tileX = tile % P.tilesX, tileY = tile / P.tilesX;
-	const int x = tileX * kTile + (wave & 1) * 8 + (lane & 7);
-	const int y = tileY * kTile + (wave >> 1) * 8 + (lane >> 3);
-	const bool active = x < P.width && y < P.height;
+	const int x = P.x0 + tileX * kTile + (wave & 1) * 8 + (lane & 7);
+	const int y = P.y0 + tileY * kTile + (wave >> 1) * 8 + (lane >> 3);
+	const bool active = x < P.x0 + P.regionW && y < P.y0 + P.regionH;
 	__shared__ StackLds stackLds;
 	const LdsStack lds = laneStack(stackLds);
+	constexpr int kTop = kBatch > 0 ? 0 : kRenderTop;
+	__shared__ float4 topLds[kTop > 0 ? 4 * kTop : 1];
+	TopNodes top{};
+	if constexpr (kTop > 0) {
+		const int n = P.R.useBvh ? min(kTop, P.R.topSlots) : 0;
+		for (int i = threadIdx.x; i < 4 * n; i += kBlock) topLds[i] = P.S.nodes[4 * (size_t) P.R.nodeBase + i];
+		__syncthreads();
+		top = TopNodes{(const LdsFloat4 *) topLds, P.R.nodeBase, n};
+	}
 	Counters<kMetrics> c;
 	c.zero();
 	if (active) {
@@ -94,7 +114,8 @@ void renderKernel(RenderParams P) {
 			uint32_t rng = path_seed(P.frameSeed, P.frameIndex, pixel, s);
 			f3 o, d;
 			cameraRay(P.cam, P.width, P.height, x, y, rng, o, d);
-			const f3 col = tracePath<kMetrics, kCap, kBatch>(P.S, P.R, o, d, P.maxDepth, P.rrStart, rng, c, lds);
+			const f3 col = tracePath<kMetrics, kCap, kBatch, kTop>(P.S, P.R, o, d, P.maxDepth, P.rrStart, rng, c, lds,
+			                                                      top);
 			acc = s == 0 ? col : acc + col;
 			if (s == P.total - 1) acc = acc * (1.0f / (float) P.total);
 		}
@@ -344,6 +365,11 @@ int mrad_create(const mrad_tracer_config *config, mrad_tracer **out) {
 	if (c.bvh_stack_size < 8 || c.bvh_stack_size > 256)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "BVH stack size must be in [8, 256]");
 	if (c.samples_per_launch <= 0 || c.samples_per_launch > c.samples_per_frame) c.samples_per_launch = c.samples_per_frame;
+	// GpuLaunchProvenance.renderTileLayout (:44-60): 1 <= pixels per launch <= 262144
+	if (c.pixels_per_launch < 0 || c.pixels_per_launch > kMaxPixelsPerLaunch)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "pixels per launch must be in [0, 262144]");
+	if (c.max_physical_kernel_nanos < 0)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "maximum physical kernel nanos must be >= 0");
 	mrad_tracer *t = new (std::nothrow) mrad_tracer();
 	if (!t) return set_error(MRAD_ERR_OUT_OF_MEMORY, "tracer allocation failed");
 	t->cfg = c;
@@ -429,19 +455,60 @@ int mrad_upload_scene(mrad_tracer *t, const mrad_scene_arrays *a, uint64_t key) 
 	return MRAD_OK;
 }
 
+// The launch plan of one frame (renderTilePlan / renderLaunchCount,
+// GpuRayTracer.java:544-564, with GpuLaunchProvenance.renderTileLayout): tiles
+// of min(W, n) x max(1, n / tile width) pixels, one launch per tile and sample
+// range; more than kMaxRenderLaunches launches is an invalid argument.  It
+// needs no device, so callers check it before any upload.
+struct LaunchPlan {
+	int tileW, tileH, tileCols, tileRows, sampleLaunches;
+	long long launches;
+};
+
+static int launchPlan(const mrad_tracer *t, LaunchPlan &L) {
+	const int W = t->cfg.width, H = t->cfg.height, spp = t->cfg.samples_per_frame, per = t->cfg.samples_per_launch;
+	const int ppl = t->cfg.pixels_per_launch;
+	L.tileW = ppl > 0 ? std::min(W, ppl) : W;
+	L.tileH = ppl > 0 ? std::min(H, std::max(1, ppl / L.tileW)) : H;
+	L.tileCols = 1 + (W - 1) / L.tileW;
+	L.tileRows = 1 + (H - 1) / L.tileH;
+	L.sampleLaunches = (spp + per - 1) / per;
+	L.launches = (long long) L.tileCols * L.tileRows * L.sampleLaunches;
+	if (L.launches > kMaxRenderLaunches)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT,
+		                 "Render would require " + std::to_string(L.launches) +
+		                     " watchdog-safe pixel/sample launches, exceeding the limit of " +
+		                     std::to_string(kMaxRenderLaunches) +
+		                     ". Increase the bounded pixel cap or reduce samples per frame.");
+	return MRAD_OK;
+}
+
+// renderLaunchCount (GpuRayTracer.java:556-564).
+int mrad_render_launch_count(const mrad_tracer *t, int *launches) {
+	clear_error();
+	if (!t || !launches) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle and output must not be null");
+	LaunchPlan L;
+	MRAD_TRY(launchPlan(t, L));
+	*launches = (int) L.launches;
+	return MRAD_OK;
+}
+
 // renderSample (GpuRayTracer.java:317-432).
 int mrad_render_sample(mrad_tracer *t, float *rgb, const float camera[12], int maxDepth, int frameIndex,
                        uint64_t frameSeed) {
 	clear_error();
 	MRAD_TRY(mrad_validate_full_path_depth(maxDepth));
 	if (!camera) return set_error(MRAD_ERR_INVALID_ARGUMENT, "GPU scene and camera must not be null");
+	if (!t) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle is null");
+	const int W = t->cfg.width, H = t->cfg.height, spp = t->cfg.samples_per_frame, per = t->cfg.samples_per_launch;
+	LaunchPlan L;
+	MRAD_TRY(launchPlan(t, L));                                     // before the device, as the reference orders it
 	MRAD_TRY(requireTracer(t));
 	std::lock_guard<std::mutex> g(t->mu);
 	if (!t->hasScene) return set_error(MRAD_ERR_ILLEGAL_STATE, "no scene is resident: call mrad_upload_scene first");
 	MRAD_HIP(hipSetDevice(t->cfg.device));
 	const int64_t frameStart = nowNanos();
-	const int W = t->cfg.width, H = t->cfg.height, spp = t->cfg.samples_per_frame, per = t->cfg.samples_per_launch;
-	const int launches = (spp + per - 1) / per;
+	const int launches = (int) L.launches;
 	MRAD_TRY(ensure(t->output, (size_t) W * H * 3 * sizeof(float)));
 	MRAD_TRY(ensure(t->stats, MRAD_STAT_COUNT * sizeof(unsigned long long)));
 	MRAD_TRY(ensureEvents(t, launches - 1));
@@ -457,20 +524,30 @@ int mrad_render_sample(mrad_tracer *t, float *rgb, const float camera[12], int m
 	P.rrStart = t->rrStart;
 	P.frameIndex = frameIndex;
 	P.total = spp;
-	P.tilesX = (W + kTile - 1) / kTile;
 	P.frameSeed = frameSeed;
 	P.out = t->output.as<float>();
 	P.stats = metrics ? t->stats.as<unsigned long long>() : nullptr;
-	const int blocks = P.tilesX * ((H + kTile - 1) / kTile);
 	RenderFn fn = metrics ? renderFor<true>(t->cfg.bvh_stack_size, t->leafBatch)
 	                      : renderFor<false>(t->cfg.bvh_stack_size, t->leafBatch);
 	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
-	for (int l = 0; l < launches; l++) {
-		P.s0 = l * per;
+	// sample ranges outermost, pixel tiles inside (the reference's order); each
+	// pixel keeps its own running sum, so the plan does not change the image
+	int l = 0;
+	for (int sl = 0; sl < L.sampleLaunches; sl++) {
+		P.s0 = sl * per;
 		P.s1 = std::min(spp, P.s0 + per);
-		hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, t->stream, P);
-		MRAD_HIP(hipGetLastError());
-		if (l + 1 < launches) MRAD_HIP(hipEventRecord(t->boundary[l], t->stream));
+		for (int ty = 0; ty < L.tileRows; ty++)
+			for (int tx = 0; tx < L.tileCols; tx++, l++) {
+				P.x0 = tx * L.tileW;
+				P.y0 = ty * L.tileH;
+				P.regionW = std::min(L.tileW, W - P.x0);
+				P.regionH = std::min(L.tileH, H - P.y0);
+				P.tilesX = (P.regionW + kTile - 1) / kTile;
+				const int blocks = P.tilesX * ((P.regionH + kTile - 1) / kTile);
+				hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, t->stream, P);
+				MRAD_HIP(hipGetLastError());
+				if (l + 1 < launches) MRAD_HIP(hipEventRecord(t->boundary[l], t->stream));
+			}
 	}
 	MRAD_HIP(hipEventRecord(t->evEnd, t->stream));
 	MRAD_HIP(hipEventSynchronize(t->evEnd));
@@ -478,11 +555,18 @@ int mrad_render_sample(mrad_tracer *t, float *rgb, const float camera[12], int m
 	MRAD_HIP(hipEventElapsedTime(&ms, t->evStart, t->evEnd));
 	int64_t maxPhys = 0;
 	hipEvent_t prev = t->evStart;
-	for (int l = 0; l < launches; l++) {
-		hipEvent_t end = l + 1 == launches ? t->evEnd : t->boundary[l];
+	for (int k = 0; k < launches; k++) {
+		hipEvent_t end = k + 1 == launches ? t->evEnd : t->boundary[k];
 		float pm = 0.0f;
 		MRAD_HIP(hipEventElapsedTime(&pm, prev, end));
-		maxPhys = std::max<int64_t>(maxPhys, (int64_t) (pm * 1.0e6f));
+		const int64_t phys = (int64_t) (pm * 1.0e6f);
+		// requireSafePhysicalKernelNanos (GpuRayTracer.java:572-580)
+		if (t->cfg.max_physical_kernel_nanos > 0 && phys >= t->cfg.max_physical_kernel_nanos)
+			return set_error(MRAD_ERR_ILLEGAL_STATE,
+			                 "Physical kernel duration must be greater than zero and less than " +
+			                     std::to_string(t->cfg.max_physical_kernel_nanos) + " ns; observed " +
+			                     std::to_string(phys) + " ns");
+		maxPhys = std::max<int64_t>(maxPhys, phys);
 		prev = end;
 	}
 	t->last.kernel_nanos = (int64_t) (ms * 1.0e6f);

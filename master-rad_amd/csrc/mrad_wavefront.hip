@@ -108,6 +108,59 @@ __device__ __forceinline__ int sceneOfPixel(const WfParams &P, uint64_t p) {
 	return lo;
 }
 
+// Order in which a scene's pixels are given path ids (each pixel's samples
+// consecutive).  1: 8x8 pixel tiles in raster order, Z-order inside a full
+// tile and row order inside the partial tiles of the right and bottom edges,
+// so the 64 paths of a trace wave cover a compact patch of the image (a 4x4
+// patch at 4 samples, 2x2 at 16) instead of a piece of one row; 0: row
+// order.  Paths of a pixel, their seeds and the per-pixel sums are the same
+// either way, so images are too.
+#ifndef MRAD_WF_PIXEL_TILES
+#define MRAD_WF_PIXEL_TILES 0
+#endif
+
+// The pixel of rank r of a width x height image in that order.
+__device__ __forceinline__ void rankToPixel(uint32_t r, int width, int height, int &x, int &y) {
+	if (!MRAD_WF_PIXEL_TILES) {
+		x = (int) (r % (uint32_t) width);
+		y = (int) (r / (uint32_t) width);
+		return;
+	}
+	const uint32_t W = (uint32_t) width, H = (uint32_t) height;
+	const uint32_t fullBands = H / 8u, bandPixels = 8u * W;
+	uint32_t band, off, bh;
+	if (r < fullBands * bandPixels) {
+		band = r / bandPixels;
+		off = r - band * bandPixels;
+		bh = 8u;
+	} else {
+		band = fullBands;
+		off = r - fullBands * bandPixels;
+		bh = H - 8u * fullBands;
+	}
+	const uint32_t fullBlocks = W / 8u, blockPixels = 8u * bh;
+	uint32_t block, in, bw;
+	if (off < fullBlocks * blockPixels) {
+		block = off / blockPixels;
+		in = off - block * blockPixels;
+		bw = 8u;
+	} else {
+		block = fullBlocks;
+		in = off - fullBlocks * blockPixels;
+		bw = W - 8u * fullBlocks;
+	}
+	uint32_t lx, ly;
+	if (bw == 8u && bh == 8u) {
+		lx = (in & 1u) | ((in >> 1) & 2u) | ((in >> 2) & 4u);
+		ly = ((in >> 1) & 1u) | ((in >> 2) & 2u) | ((in >> 3) & 4u);
+	} else {
+		lx = in % bw;
+		ly = in / bw;
+	}
+	x = (int) (8u * block + lx);
+	y = (int) (8u * band + ly);
+}
+
 // Paths the next generate step appends to pool `cur`: up to the pool's free
 // space, from the current chunk only, and none from a chunk the host has not
 // released yet.
@@ -136,8 +189,10 @@ __global__ __launch_bounds__(kBlock) void wfGenerate(WfParams P, int cur) {
 		const int k = sceneOfPath(P, g);
 		const WfScene &V = P.scenes[k];
 		const uint64_t local = g - V.pathBase;
-		const uint32_t pixel = (uint32_t) (local / (uint64_t) V.spp), s = (uint32_t) (local % (uint64_t) V.spp);
-		const int x = (int) (pixel % (uint32_t) V.width), y = (int) (pixel / (uint32_t) V.width);
+		const uint32_t rank = (uint32_t) (local / (uint64_t) V.spp), s = (uint32_t) (local % (uint64_t) V.spp);
+		int x, y;
+		rankToPixel(rank, V.width, V.height, x, y);
+		const uint32_t pixel = (uint32_t) y * (uint32_t) V.width + (uint32_t) x;
 		uint32_t rng = path_seed(V.frameSeed, V.frameIndex, pixel, (int) s);
 		f3 o, d;
 		cameraRay(V.cam, V.width, V.height, x, y, rng, o, d);
@@ -219,23 +274,50 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 #define MRAD_TRACE_WAVES_PER_EU 7
 #endif
 
+// Node slots of a scene's top BVH levels each trace wave keeps in LDS (0:
+// none).  A wave reloads them when the scene of its chunk's first path
+// changes; paths of other scenes in the chunk read every node from HBM.
+#ifndef MRAD_TRACE_TOP
+#define MRAD_TRACE_TOP 0
+#endif
+constexpr int kTraceTop = MRAD_TRACE_TOP;
+
 // Closest hit of every live path in pool `cur` (persistent).
 template <bool kMetrics, int kCap, int kShort>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
 void wfTrace(WfParams P, int cur) {
 	__shared__ StackLdsT<kShort> stackLds;
 	const LdsStack lds = laneStack(stackLds);
+	constexpr int kTop = kTraceTop;
+	__shared__ float4 topLds[kWaves][kTop > 0 ? 4 * kTop : 1];
 	const uint32_t count = P.counters[kCount0 + cur];
 	const WfPool Q = P.pool[cur];
-	const int lane = threadIdx.x & 63;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int cached = -1;
+	TopNodes top{(const LdsFloat4 *) topLds[wave], 0, 0};
 	Counters<kMetrics> c;
 	c.zero();
 	for (;;) {
 		const uint32_t base = fetchChunk(P.counters + kWorkTrace);
 		if (base >= count) break;
 		const uint32_t i = base + lane;
+		uint4 meta = make_uint4(0, 0, 0xFFFFFFFFu, 0);
+		if (i < count) meta = Q.meta[i];
+		if constexpr (kTop > 0) {
+			// lane 0's path (base < count) names the scene staged for the chunk
+			const int lead = __shfl((int) (meta.z & 0xFFFFu), 0, 64);
+			if (lead != cached) {
+				const SceneRange &L = P.scenes[lead].R;
+				const int n = L.useBvh ? min(kTop, L.topSlots) : 0;
+				for (int k = lane; k < 4 * n; k += 64) topLds[wave][k] = P.S.nodes[4 * (size_t) L.nodeBase + k];
+				__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				cached = lead;
+				top.base = L.nodeBase;
+				top.count = n;
+			}
+		}
 		if (i < count) {
-			const uint4 meta = Q.meta[i];
 			if ((meta.z >> 16) == kDead) {
 				P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 				continue;
@@ -245,7 +327,7 @@ void wfTrace(WfParams P, int cur) {
 			const SceneRange &R = P.scenes[meta.z & 0xFFFFu].R;
 			c.add(0, 1);
 			Best best;
-			const bool found = traceClosest<kMetrics, kCap, kShort>(P.S, R, o, d, best, c, lds);
+			const bool found = traceClosest<kMetrics, kCap, kShort, 0, kTop>(P.S, R, o, d, best, c, lds, top);
 			writeHit(P, i, o, d, best, found);
 		}
 	}
@@ -389,10 +471,10 @@ __global__ __launch_bounds__(kBlock) void wfResolve(WfParams P, uint32_t c) {
 	const WfChunk C = P.chunks[c];
 	const size_t slotBase = (size_t) (c & 1u) * P.chunkPaths;
 	for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < C.pixelCount; j += gridDim.x * kBlock) {
-		const uint64_t p = C.pixelStart + j;
+		const uint64_t p = C.pixelStart + j;        // the pixel of this rank in the batch's rank order
 		const WfScene &V = P.scenes[sceneOfPixel(P, p)];
-		const uint64_t pixel = p - V.pixelBase;
-		const size_t first = slotBase + (size_t) (V.pathBase + pixel * (uint64_t) V.spp - C.pathStart);
+		const uint64_t rank = p - V.pixelBase;
+		const size_t first = slotBase + (size_t) (V.pathBase + rank * (uint64_t) V.spp - C.pathStart);
 		const float *r = P.results + 3 * first;
 		f3 acc = mk(0.0f, 0.0f, 0.0f);
 		if ((V.spp & 3) == 0 && (first & 3) == 0) {
@@ -414,9 +496,12 @@ __global__ __launch_bounds__(kBlock) void wfResolve(WfParams P, uint32_t c) {
 				if (s == V.spp - 1) acc = acc * (1.0f / (float) V.spp);
 			}
 		}
-		P.images[3 * (size_t) p] = acc.x;
-		P.images[3 * (size_t) p + 1] = acc.y;
-		P.images[3 * (size_t) p + 2] = acc.z;
+		int x, y;
+		rankToPixel((uint32_t) rank, V.width, V.height, x, y);
+		const size_t out = 3 * ((size_t) V.pixelBase + (size_t) y * (size_t) V.width + (size_t) x);
+		P.images[out] = acc.x;
+		P.images[out + 1] = acc.y;
+		P.images[out + 2] = acc.z;
 	}
 }
 

@@ -63,7 +63,8 @@ class SceneArrays(C.Structure):
 class TracerConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples_per_frame", C.c_int),
                 ("bvh_traversal", C.c_int), ("collect_metrics", C.c_int), ("bvh_stack_size", C.c_int),
-                ("samples_per_launch", C.c_int), ("device", C.c_int)]
+                ("samples_per_launch", C.c_int), ("device", C.c_int), ("pixels_per_launch", C.c_int),
+                ("max_physical_kernel_nanos", C.c_int64)]
 
 
 class FrameStats(C.Structure):
@@ -93,6 +94,7 @@ SIGNATURES = {
     "mrad_set_russian_roulette": (C.c_int, [C.c_void_p, C.c_int]),
     "mrad_set_leaf_batch": (C.c_int, [C.c_void_p, C.c_int]),
     "mrad_render_sample": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64]),
+    "mrad_render_launch_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mrad_copy_frame": (C.c_int, [C.c_void_p, fp]),
     "mrad_trace_replay": (C.c_int, [C.c_void_p, fp, C.c_int, C.c_int, ip, fp, ip, ip, fp, up]),
     "mrad_trace_replay_noalloc": (C.c_int, [C.c_void_p, fp, C.c_int, C.c_int, ip, fp, ip, ip, fp, up]),

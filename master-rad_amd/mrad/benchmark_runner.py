@@ -197,14 +197,25 @@ def base_configuration(phase: str, process_id: int, width: int, height: int, max
             f"seed:0x{seed & 0xFFFFFFFFFFFFFFFF:x},python:{platform.python_version()}")
 
 
-def gpu_configuration(width: int, height: int, samples_per_frame: int) -> str:
+MAX_RENDER_PIXELS_PER_LAUNCH = 262_144      # GpuLaunchProvenance.java:5
+
+
+def render_tile_count(width: int, height: int, pixels_per_launch: int) -> int:
+    """GpuLaunchProvenance.renderTileLayout (:44-60): tiles of min(W, n) x max(1, n / tile width) pixels."""
+    tw = min(width, pixels_per_launch)
+    th = min(height, max(1, pixels_per_launch // tw))
+    return (1 + (width - 1) // tw) * (1 + (height - 1) // th)
+
+
+def gpu_configuration(width: int, height: int, samples_per_frame: int,
+                      pixels_per_launch: int = MAX_RENDER_PIXELS_PER_LAUNCH) -> str:
     """The launch provenance appended for GPU versions (BenchmarkRunner.java:48-59).  As in the reference
-    (GpuLaunchProvenance.SAMPLES_PER_PHYSICAL_KERNEL = 1, physicalRenderLaunches = tiles x samples), each
-    physical launch renders one sample; one launch covers the whole image (one tile), so a logical frame is
-    samples_per_frame launches.  run() builds its renderers with samples_per_launch = 1 to match."""
-    launches = SAMPLES_PER_PHYSICAL_KERNEL * samples_per_frame
-    return (f",logicalSamplesPerFrame:{samples_per_frame},renderPixelsPerLaunch:{width * height},"
-            f"replayRaysPerLaunch:{width * height},samplesPerPhysicalKernel:{SAMPLES_PER_PHYSICAL_KERNEL},"
+    (GpuLaunchProvenance: SAMPLES_PER_PHYSICAL_KERNEL = 1, physicalRenderLaunches = tiles x samples), each
+    physical launch renders one sample of one pixel tile of at most renderPixelsPerLaunch pixels; run()
+    builds its renderers with samples_per_launch = 1 and that tile size to match."""
+    launches = render_tile_count(width, height, pixels_per_launch) * samples_per_frame
+    return (f",logicalSamplesPerFrame:{samples_per_frame},renderPixelsPerLaunch:{pixels_per_launch},"
+            f"replayRaysPerLaunch:{MAX_RENDER_PIXELS_PER_LAUNCH},samplesPerPhysicalKernel:{SAMPLES_PER_PHYSICAL_KERNEL},"
             f"physicalKernelLaunchesPerLogicalFrame:{launches},kernelNanosSemantics:{KERNEL_NANOS_SEMANTICS}")
 
 
@@ -276,6 +287,9 @@ class Plan:
         self.samples_per_frame = required_positive_int(props, "gfxlab.gpu.samplesPerFrame")
         self.collect_metrics = java_boolean(props, "gfxlab.gpu.collectMetrics")
         self.stack_size = optional_int(props, "gfxlab.gpu.bvhStackSize", 32)
+        self.pixels_per_launch = optional_int(props, "gfxlab.gpu.renderPixelsPerLaunch", MAX_RENDER_PIXELS_PER_LAUNCH)
+        if not 1 <= self.pixels_per_launch <= MAX_RENDER_PIXELS_PER_LAUNCH:
+            raise _invalid(f"gfxlab.gpu.renderPixelsPerLaunch must be between 1 and {MAX_RENDER_PIXELS_PER_LAUNCH}")
         if not 8 <= self.stack_size <= 256:
             raise _invalid("gfxlab.gpu.bvhStackSize must be between 8 and 256")
         self.leaf_batch = optional_int(props, "gfxlab.gpu.leafBatch", 0)
@@ -288,7 +302,7 @@ class Plan:
         self.configs = {v: bvh_config(props, v, self.leaf_size) for v in self.versions}
 
     def configuration(self, scene_name: str = "") -> str:
-        text = self.base + gpu_configuration(self.width, self.height, self.samples_per_frame)
+        text = self.base + gpu_configuration(self.width, self.height, self.samples_per_frame, self.pixels_per_launch)
         return text + f",layoutId:{self.layout_id}" if scene_name.startswith("f") else text
 
 
@@ -313,7 +327,8 @@ def run(plan: Plan, out=None, err=None) -> int:
                                                    plan.seed, plan.samples_per_frame, version != "GPU_LINEAR",
                                                    collect_metrics=plan.collect_metrics,
                                                    bvh_stack_size=plan.stack_size,
-                                                   samples_per_launch=SAMPLES_PER_PHYSICAL_KERNEL)
+                                                   samples_per_launch=SAMPLES_PER_PHYSICAL_KERNEL,
+                                                   pixels_per_launch=plan.pixels_per_launch)
                     try:
                         if plan.leaf_batch:
                             renderer.tracer.set_leaf_batch(plan.leaf_batch)

@@ -108,9 +108,12 @@ class GpuRayTracer:
 
     def __init__(self, width: int, height: int, samples_per_frame: int, bvh_traversal: bool,
                  collect_metrics: bool = False, bvh_stack_size: int = 32, samples_per_launch: int = 0,
-                 device: int = 0):
+                 device: int = 0, pixels_per_launch: int = 0, max_physical_kernel_nanos: int = 0):
+        """pixels_per_launch is gfxlab.gpu.renderPixelsPerLaunch (0: the whole image per launch) and
+        max_physical_kernel_nanos the per-launch duration guard (0: none; the reference uses 250 ms)."""
         cfg = N.TracerConfig(int(width), int(height), int(samples_per_frame), 1 if bvh_traversal else 0,
-                             1 if collect_metrics else 0, int(bvh_stack_size), int(samples_per_launch), int(device))
+                             1 if collect_metrics else 0, int(bvh_stack_size), int(samples_per_launch), int(device),
+                             int(pixels_per_launch), int(max_physical_kernel_nanos))
         h = C.c_void_p()
         N.check(N.lib.mrad_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -162,10 +165,18 @@ class GpuRayTracer:
                                         f"RGB output must contain at least {self.width * self.height * 3} floats")
         if scene is None or camera is None:
             raise N.InvalidArgument(N.MRAD_ERR_INVALID_ARGUMENT, "GPU scene and camera must not be null")
+        self.render_launch_count()
         self.ensure_scene_resident(scene)
         cam = camera.packed() if isinstance(camera, GpuCamera) else np.ascontiguousarray(camera, np.float32)
         N.check(N.lib.mrad_render_sample(self._handle(), N.ptr(rgb, N.fp), N.ptr(cam, N.fp), int(max_depth),
                                          int(frame_index), C.c_uint64(frame_seed & ((1 << 64) - 1))))
+
+    def render_launch_count(self) -> int:
+        """renderLaunchCount (GpuRayTracer.java:556-564): pixel tiles x sample ranges of one frame; more than
+        8192 raises InvalidArgument.  Needs no device."""
+        n = C.c_int()
+        N.check(N.lib.mrad_render_launch_count(self._handle(), C.byref(n)))
+        return n.value
 
     def condition_steady_state_sample(self, scene: GpuScene, camera: GpuCamera, max_depth: int, frame_index: int,
                                       frame_seed: int) -> ConditioningStats:

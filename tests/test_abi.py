@@ -157,3 +157,34 @@ def test_batch_accepts_more_than_2_32_paths_and_checks_view_limits():
     for bad in (-1, (1 << 30) + 1):
         with pytest.raises(mrad.InvalidArgument):
             mrad.BatchRenderer(mrad.WavefrontConfig(result_chunk_paths=bad))
+
+
+def test_render_launch_plan_limits_are_checked_before_the_device():
+    """renderLaunchCount (GpuRayTracer.java:550-564): a plan of more than 8192 pixel-tile x sample launches is an
+    invalid argument even where no device is present; pixels per launch outside [0, 262144] and a negative
+    watchdog limit are rejected at creation."""
+    sc = mrad.GpuScene()
+    sc.add_spheres([[0, 0, 0, 1]], mrad.material_array((0.5, 0.5, 0.5)))
+    cam = mrad.GpuCamera.look_at((0, 0, -3), (0, 0, 0), 40)
+    t = mrad.GpuRayTracer(128, 128, 1, True, pixels_per_launch=1)          # 16384 one-pixel tiles
+    with pytest.raises(mrad.InvalidArgument, match="require 16384 watchdog-safe .* exceeding the limit of 8192"):
+        t.render_sample(np.zeros(128 * 128 * 3, np.float32), sc, cam, 4, 0, 1)
+    with pytest.raises(mrad.InvalidArgument, match="exceeding the limit of 8192"):
+        t.render_launch_count()
+    t.close()
+    # 8192 launches exactly is allowed: 64 x 128 two-pixel tiles of one sample each
+    t = mrad.GpuRayTracer(128, 128, 1, True, pixels_per_launch=2)
+    assert t.render_launch_count() == 8192
+    t.close()
+    # tiles x sample launches, against the provenance layout rule
+    from mrad.benchmark_runner import render_tile_count
+    for w, h, spp, per, ppl in ((1920, 1080, 4, 1, 262_144), (1920, 1080, 4, 2, 262_144), (640, 480, 3, 0, 0),
+                                (1000, 7, 5, 1, 333), (7, 1000, 2, 1, 5)):
+        t = mrad.GpuRayTracer(w, h, spp, True, samples_per_launch=per, pixels_per_launch=ppl)
+        tiles = render_tile_count(w, h, ppl) if ppl else 1
+        ranges = 1 if per == 0 else -(-spp // per)
+        assert t.render_launch_count() == tiles * ranges, (w, h, spp, per, ppl)
+        t.close()
+    for kw in (dict(pixels_per_launch=262_145), dict(pixels_per_launch=-1), dict(max_physical_kernel_nanos=-5)):
+        with pytest.raises(mrad.InvalidArgument):
+            mrad.GpuRayTracer(8, 8, 1, True, **kw)

@@ -31,7 +31,7 @@ def test_plan_from_properties():
     text = plan.configuration()
     assert text.startswith("phase:timing,processId:2,width:64,height:32,maxDepth:12,bvhLeafSize:8,"
                            "warmupFrames:1,measuredFrames:3,seed:0x5eed,python:")
-    assert text.endswith(",logicalSamplesPerFrame:4,renderPixelsPerLaunch:2048,replayRaysPerLaunch:2048,"
+    assert text.endswith(",logicalSamplesPerFrame:4,renderPixelsPerLaunch:262144,replayRaysPerLaunch:262144,"
                          "samplesPerPhysicalKernel:1,physicalKernelLaunchesPerLogicalFrame:4,"
                          "kernelNanosSemantics:" + B.KERNEL_NANOS_SEMANTICS)
     assert plan.configs["GPU_BVH"] == mrad.BvhBuildConfig.uniform(8)
@@ -131,3 +131,12 @@ def test_failed_pairs_exit_with_status_2(tmp_path, capsys):
     captured = capsys.readouterr()
     assert "=== Running GPU_BVH on scene hero-96" in captured.out
     assert "Failed to run GPU_BVH on hero-96:" in captured.err
+
+
+def test_launch_plan_is_the_reference_tile_layout():
+    """GpuLaunchProvenance.renderTileLayout: a 1920x1080 frame at the default 262,144 pixels per launch is 8
+    tiles of 1920x136 rows, so one sample is 8 physical launches; a cap of 1000 pixels gives 2 x 1080 tiles."""
+    assert B.render_tile_count(1920, 1080, 262_144) == 8
+    assert B.render_tile_count(1920, 1080, 1000) == 2 * 1080
+    assert B.render_tile_count(64, 32, 262_144) == 1
+    assert "physicalKernelLaunchesPerLogicalFrame:16," in B.gpu_configuration(1920, 1080, 2)

@@ -129,6 +129,31 @@ def test_render_split_launches_equal(require_gpu):
     np.testing.assert_array_equal(out[0], out[2])
 
 
+def test_render_pixel_tiles_equal_one_launch(require_gpu):
+    """The reference's launch plan (pixel tiles of at most renderPixelsPerLaunch pixels, one sample per launch,
+    GpuRayTracer.java:317-432) renders the same frame and counters as one launch, for tile sizes that do and do
+    not divide the frame; a per-launch duration limit below the launch time fails the frame with IllegalState."""
+    scene, cam = configs.room_scene(), configs.room_camera()
+    w, h, spp = 70, 45, 3
+    ref, ref_stats = None, None
+    for ppl, per in ((0, 0), (1000, 1), (70 * 9, 2), (1, 3)):
+        with mrad.GpuRayTracer(w, h, spp, True, collect_metrics=True, samples_per_launch=per,
+                               pixels_per_launch=ppl) as t:
+            img = np.zeros(w * h * 3, np.float32)
+            t.render_sample(img, scene, cam, 4, 0, 77)
+            st = t.last_frame_stats()
+        if ref is None:
+            ref, ref_stats = img, st
+            continue
+        np.testing.assert_array_equal(img.view(np.uint32), ref.view(np.uint32))
+        assert all(st[k] == ref_stats[k] for k in mrad.STAT_NAMES)
+        tiles = {1000: 4, 70 * 9: 5, 1: w * h}[ppl]
+        assert st["physical_launches"] == tiles * (-(-spp // (per or spp)))
+    with mrad.GpuRayTracer(w, h, spp, True, max_physical_kernel_nanos=1) as t:
+        with pytest.raises(mrad.IllegalState, match="less than 1 ns"):
+            t.render_sample(np.zeros(w * h * 3, np.float32), scene, cam, 4, 0, 77)
+
+
 def test_full_size_field_sampled_pixels(require_gpu):
     """het-100k at 1920x1080: sampled pixels of the full GPU frame equal the oracle's."""
     w = configs.het(100_000, spp=2)
