@@ -17,6 +17,12 @@ step so a profiled run shows progress.
 averages the counters of every dispatch whose kernel name matches
 KERNEL_REGEX over the rocprofv3 counter_collection.csv files given (one file
 per counter pass) and writes them, per dispatch, to OUT.json.
+
+    python -m mrad.profile_run traffic OUT.json KERNEL_REGEX WORKLOAD FETCH.csv WRITE.csv COPY_FETCH.csv COPY_WRITE.csv
+
+turns FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch) into HBM bytes per
+launch of the matching kernel, each counter scaled by what it reads for the
+1 GiB device copy of `copy` (bytes moved / counter reading).
 """
 from __future__ import annotations
 
@@ -155,10 +161,42 @@ def summarize(out: str, kernel_regex: str, csvs: list[str]) -> dict:
     return res
 
 
+def traffic(out: str, kernel_regex: str, workload: str, fetch_csv: str, write_csv: str, copy_fetch_csv: str,
+            copy_write_csv: str, copy_bytes: int = 1 << 30) -> dict:
+    """HBM bytes per launch from one FETCH_SIZE and one WRITE_SIZE counter run of the workload and the same two
+    counters on a plain device copy of copy_bytes (which reads and writes copy_bytes once each)."""
+    import json
+    import tempfile
+    from pathlib import Path
+
+    def mean(csv_path: str, regex: str, counter: str) -> tuple[float, int, list]:
+        with tempfile.TemporaryDirectory() as d:
+            r = summarize(str(Path(d) / "s.json"), regex, [csv_path])
+        return r["per_dispatch"][counter], r["dispatches"][counter], r["kernels"]
+
+    fetch, n, kernels = mean(fetch_csv, kernel_regex, "FETCH_SIZE")
+    write, _, _ = mean(write_csv, kernel_regex, "WRITE_SIZE")
+    copy_fetch, _, _ = mean(copy_fetch_csv, "copyBuffer", "FETCH_SIZE")
+    copy_write, _, _ = mean(copy_write_csv, "copyBuffer", "WRITE_SIZE")
+    fetch_scale = copy_bytes / 1024.0 / copy_fetch
+    write_scale = copy_bytes / 1024.0 / copy_write
+    res = {"kernel": kernels[0] if len(kernels) == 1 else kernels, "workload": workload,
+           "fetch_kib_per_dispatch": fetch, "write_kib_per_dispatch": write,
+           "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "hbm_bytes_per_launch": (fetch * fetch_scale + write * write_scale) * 1024.0, "dispatches": n}
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def main(argv):
     what = argv[1] if len(argv) > 1 else "het"
     if what == "summarize":
         summarize(argv[2], argv[3], argv[4:])
+        return
+    if what == "traffic":
+        traffic(*argv[2:9])
         return
     if what == "het":
         _het(int(argv[2]) if len(argv) > 2 else 16)
