@@ -131,11 +131,11 @@ def _cpu_run(index: int, width: int, height: int, spp: int, threads: int, frames
 def cpu_baseline(w: configs.Workload, index: int, processes: int = 3) -> dict:
     """FP64 CPU restatement of the reference CPU renderer (parallel rows, oracle/oracle_fp64.c) timed as the
     reference's protocol times it: `processes` fresh processes, each one warm-up and three timed frames, the
-    median of its frames, and the median over processes.  The sample is the same scene and camera at half
-    resolution in each axis (960x540 of a 1080p view, 4 spp), on this process's CPUs (at most 16), and one
-    process on one thread at 240x135 for the single-thread figure."""
+    median of its frames, and the median over processes.  Each frame is one sample per pixel of the whole view
+    (RendererFactory.java:46-51: the CPU renderers accumulate 1-spp frames), on this process's CPUs (at most 16),
+    and one process on one thread at a quarter of the resolution in each axis for the single-thread figure."""
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    sw, sh, spp = max(8, w.width // 2), max(8, w.height // 2), 4
+    sw, sh, spp = w.width, w.height, 1
     per_proc = []
     for _ in range(processes):
         r = _cpu_run(index, sw, sh, spp, threads, 3, 600)
@@ -143,7 +143,7 @@ def cpu_baseline(w: configs.Workload, index: int, processes: int = 3) -> dict:
         log(f"cpu baseline process: frames {['%.2f' % t for t in r['frame_seconds']]} s")
     med = float(np.median(per_proc))
     paths = sw * sh * spp
-    qw, qh = max(8, w.width // 8), max(8, w.height // 8)
+    qw, qh = max(8, w.width // 4), max(8, w.height // 4)
     r1 = _cpu_run(index, qw, qh, spp, 1, 3, 600)
     t1 = float(np.median(r1["frame_seconds"]))
     return {"value": paths / med / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",

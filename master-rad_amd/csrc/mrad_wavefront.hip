@@ -55,11 +55,14 @@ struct WfChunk {
 };
 
 // Path pool, structure of arrays of 16-byte records:
-//   od0 = (o.x, o.y, o.z, d.x)   od1 = (d.y, d.z, T.x, T.y)
-//   tl  = (T.z, L.x, L.y, L.z)   meta = (rng, result slot, scene | depth << 16, 0)
+//   ori = (o.x, o.y, o.z, 0)     dir  = (d.x, d.y, d.z, T.x)
+//   tl  = (T.y, T.z, L.x, L.y)   meta = (rng, result slot, scene | depth << 16, L.z bits)
 // The result slot is buffer * chunkPaths + the path's index in its chunk.
+// Trace reads ori, dir and meta; shade reads dir and meta, and tl only past
+// depth 0 (a camera path starts with T = 1 and L = 0, which generate does not
+// store); nothing but trace reads the origin.
 struct WfPool {
-	float4 *od0, *od1, *tl;
+	float4 *ori, *dir, *tl;
 	uint4 *meta;
 };
 
@@ -205,10 +208,9 @@ __global__ __launch_bounds__(kBlock) void wfGenerate(WfParams P, int cur) {
 			P.results[3 * (size_t) result + 2] = 0.0f;
 		}
 		const uint32_t slot = base + j;
-		Q.od0[slot] = make_float4(o.x, o.y, o.z, d.x);
-		Q.od1[slot] = make_float4(d.y, d.z, 1.0f, 1.0f);
-		Q.tl[slot] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-		Q.meta[slot] = make_uint4(rng, result, (uint32_t) k | (depth << 16), 0u);
+		Q.ori[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+		Q.dir[slot] = make_float4(d.x, d.y, d.z, 1.0f);
+		Q.meta[slot] = make_uint4(rng, result, (uint32_t) k | (depth << 16), __float_as_uint(0.0f));
 	}
 }
 
@@ -282,17 +284,32 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 #endif
 constexpr int kTraceTop = MRAD_TRACE_TOP;
 
-// Closest hit of every live path in pool `cur` (persistent).
-template <bool kMetrics, int kCap, int kShort>
+// Octant of a direction: bit 0 for x < 0, bit 1 for y < 0, bit 2 for z < 0.
+__device__ __forceinline__ int octantOf(float dx, float dy, float dz) {
+	return (dx < 0.0f ? 1 : 0) | (dy < 0.0f ? 2 : 0) | (dz < 0.0f ? 4 : 0);
+}
+
+// Closest hit of every live path in pool `cur` (persistent).  With kFuse the
+// same kernel also shades each path once the wave's 64 traversals have ended
+// (all lanes active, the traversal state dead) and appends the survivors to
+// pool cur ^ 1, grouped by the octant of their new direction within the wave,
+// with one atomic per wave on the destination count; finished paths write
+// their radiance and are counted per result buffer as wfShade counts them.
+// The hit records are then never written.
+template <bool kMetrics, int kCap, int kShort, bool kFuse = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
 void wfTrace(WfParams P, int cur) {
 	__shared__ StackLdsT<kShort> stackLds;
 	const LdsStack lds = laneStack(stackLds);
 	constexpr int kTop = kTraceTop;
 	__shared__ float4 topLds[kWaves][kTop > 0 ? 4 * kTop : 1];
+	__shared__ uint32_t doneLds[2];
 	const uint32_t count = P.counters[kCount0 + cur];
 	const WfPool Q = P.pool[cur];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	uint32_t finished[2] = {0u, 0u};
+	if (kFuse && threadIdx.x < 2) doneLds[threadIdx.x] = 0u;
 	int cached = -1;
 	TopNodes top{(const LdsFloat4 *) topLds[wave], 0, 0};
 	Counters<kMetrics> c;
@@ -317,29 +334,106 @@ void wfTrace(WfParams P, int cur) {
 				top.count = n;
 			}
 		}
-		if (i < count) {
-			if ((meta.z >> 16) == kDead) {
-				P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-				continue;
+		if constexpr (!kFuse) {
+			if (i < count) {
+				if ((meta.z >> 16) == kDead) {
+					P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+					continue;
+				}
+				const float4 a = Q.ori[i], b = Q.dir[i];
+				const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+				const SceneRange &R = P.scenes[meta.z & 0xFFFFu].R;
+				c.add(0, 1);
+				Best best;
+				const bool found = traceClosest<kMetrics, kCap, kShort, 0, kTop>(P.S, R, o, d, best, c, lds, top);
+				writeHit(P, i, o, d, best, found);
 			}
-			const float4 a = Q.od0[i], b = Q.od1[i];
-			const f3 o = mk(a.x, a.y, a.z), d = mk(a.w, b.x, b.y);
-			const SceneRange &R = P.scenes[meta.z & 0xFFFFu].R;
-			c.add(0, 1);
-			Best best;
-			const bool found = traceClosest<kMetrics, kCap, kShort, 0, kTop>(P.S, R, o, d, best, c, lds, top);
-			writeHit(P, i, o, d, best, found);
+		} else {
+			bool alive = false;
+			float4 n0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), n1 = n0, n2 = n0;
+			if (i < count) {
+				const uint32_t depth = meta.z >> 16;
+				const uint32_t buffer = meta.y >= P.chunkPaths ? 1u : 0u;
+				if (depth == kDead) {
+					finished[buffer]++;
+				} else {
+					const float4 a = Q.ori[i], b = Q.dir[i];
+					f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
+					const WfScene &V = P.scenes[meta.z & 0xFFFFu];
+					c.add(0, 1);
+					Best best;
+					const bool found =
+					    traceClosest<kMetrics, kCap, kShort, 0, kTop>(P.S, V.R, o, d, best, c, lds, top);
+					HitDetail h{};
+					if (found) h = resolveHit(P.S, o, d, best.t, best.kind, best.index);
+					const float4 t4 = depth == 0 ? make_float4(1.0f, 1.0f, 0.0f, 0.0f) : Q.tl[i];
+					f3 T = mk(b.w, t4.x, t4.y), L = mk(t4.z, t4.w, __uint_as_float(meta.w));
+					uint32_t rng = meta.x;
+					alive = shadeResolved(P.S, V.R.bg, found, h, (int) depth, V.maxDepth, V.rrStart, o, d, T, L,
+					                      rng) &&
+					        (int) depth + 1 < V.maxDepth;
+					if (alive) {
+						n0 = make_float4(o.x, o.y, o.z, 0.0f);
+						n1 = make_float4(d.x, d.y, d.z, T.x);
+						n2 = make_float4(T.y, T.z, L.x, L.y);
+						meta.x = rng;
+						meta.z = (meta.z & 0xFFFFu) | ((depth + 1) << 16);
+						meta.w = __float_as_uint(L.z);
+					} else {
+						float *r = P.results + 3 * (size_t) meta.y;
+						r[0] = L.x;
+						r[1] = L.y;
+						r[2] = L.z;
+						finished[buffer]++;
+					}
+				}
+			}
+			// survivors of the wave, stable within each octant of the new direction
+			const int bucket = alive ? octantOf(n1.x, n1.y, n1.z) : 0;
+			uint32_t before = 0, total = 0;
+			unsigned long long mine = 0ull;
+			for (int q = 0; q < 8; q++) {
+				const unsigned long long m = __ballot(alive && bucket == q);
+				if (q == bucket) {
+					mine = m;
+					before = total;
+				}
+				total += (uint32_t) __popcll(m);
+			}
+			uint32_t dest = 0;
+			if (lane == 0 && total) dest = atomicAdd(P.counters + kCount0 + (cur ^ 1), total);
+			dest = __shfl(dest, 0, 64);
+			if (alive) {
+				const WfPool D = P.pool[cur ^ 1];
+				const uint32_t slot = dest + before + (uint32_t) __popcll(mine & below);
+				D.ori[slot] = n0;
+				D.dir[slot] = n1;
+				D.tl[slot] = n2;
+				D.meta[slot] = meta;
+			}
 		}
 	}
 	if (P.stats) flushCounters<kMetrics>(c, P.stats);
+	if constexpr (kFuse) {
+		// finished paths per result buffer: one LDS sum per block, one atomic
+		// per block and buffer
+		__syncthreads();
+		for (int b = 0; b < 2; b++) {
+			uint32_t v = finished[b];
+			for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+			if (lane == 0 && v) atomicAdd(&doneLds[b], v);
+		}
+		__syncthreads();
+		if (threadIdx.x < 2 && doneLds[threadIdx.x]) atomicAdd(P.counters + kDone0 + threadIdx.x, doneLds[threadIdx.x]);
+	}
 }
 
 // Shading of one pool entry; true when the path continues (its new state is
-// left in od0, od1, tl, meta), false when it finished (its radiance is written
+// left in ori, dir, tl, meta), false when it finished (its radiance is written
 // to its result slot) or the entry is empty.  A path that finishes here, or
 // that never traced (depth 0), adds one to finished[its result buffer].
-__device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, uint32_t i, uint32_t count, float4 &od0,
-                                           float4 &od1, float4 &tl, uint4 &meta, uint32_t (&finished)[2]) {
+__device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, uint32_t i, uint32_t count, float4 &ori,
+                                           float4 &dir, float4 &tl, uint4 &meta, uint32_t (&finished)[2]) {
 	if (i >= count) return false;
 	meta = Q.meta[i];
 	const uint32_t depth = meta.z >> 16;
@@ -348,11 +442,10 @@ __device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, u
 		finished[buffer]++;
 		return false;
 	}
-	od0 = Q.od0[i];
-	od1 = Q.od1[i];
-	tl = Q.tl[i];
-	f3 o = mk(od0.x, od0.y, od0.z), d = mk(od0.w, od1.x, od1.y);
-	f3 T = mk(od1.z, od1.w, tl.x), L = mk(tl.y, tl.z, tl.w);
+	dir = Q.dir[i];
+	tl = depth == 0 ? make_float4(1.0f, 1.0f, 0.0f, 0.0f) : Q.tl[i];
+	f3 o, d = mk(dir.x, dir.y, dir.z);          // o: the new origin (the incoming one is not needed)
+	f3 T = mk(dir.w, tl.x, tl.y), L = mk(tl.z, tl.w, __uint_as_float(meta.w));
 	uint32_t rng = meta.x;
 	const WfScene &V = P.scenes[meta.z & 0xFFFFu];
 	const float4 ha = P.hitA[i], hb = P.hitB[i];
@@ -364,11 +457,12 @@ __device__ __forceinline__ bool shadeEntry(const WfParams &P, const WfPool &Q, u
 	const bool alive = shadeResolved(P.S, V.R.bg, found, h, (int) depth, V.maxDepth, V.rrStart, o, d, T, L, rng) &&
 	                   (int) depth + 1 < V.maxDepth;
 	if (alive) {
-		od0 = make_float4(o.x, o.y, o.z, d.x);
-		od1 = make_float4(d.y, d.z, T.x, T.y);
-		tl = make_float4(T.z, L.x, L.y, L.z);
+		ori = make_float4(o.x, o.y, o.z, 0.0f);
+		dir = make_float4(d.x, d.y, d.z, T.x);
+		tl = make_float4(T.y, T.z, L.x, L.y);
 		meta.x = rng;
 		meta.z = (meta.z & 0xFFFFu) | ((depth + 1) << 16);
+		meta.w = __float_as_uint(L.z);
 	} else {
 		float *r = P.results + 3 * (size_t) meta.y;
 		r[0] = L.x;
@@ -413,17 +507,17 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 	uint32_t finished[2] = {0u, 0u};
 	if (threadIdx.x < 2) doneLds[threadIdx.x] = 0u;
 	for (uint32_t tile = blockIdx.x * kShadeTile; tile < count; tile += gridDim.x * kShadeTile) {
-		float4 od0[kShadePer], od1[kShadePer], tl[kShadePer];
+		float4 ori[kShadePer], dir[kShadePer], tl[kShadePer];
 		uint4 meta[kShadePer];
 		bool alive[kShadePer];
 		int bucket[kShadePer];
 		unsigned long long mask[kShadePer];
 #pragma unroll
 		for (int k = 0; k < kShadePer; k++) {
-			alive[k] = shadeEntry(P, Q, tile + k * kBlock + threadIdx.x, count, od0[k], od1[k], tl[k], meta[k],
+			alive[k] = shadeEntry(P, Q, tile + k * kBlock + threadIdx.x, count, ori[k], dir[k], tl[k], meta[k],
 			                      finished);
 			bucket[k] = (kBuckets == 1 || !alive[k]) ? 0
-			          : (od0[k].w < 0.0f ? 1 : 0) | (od1[k].x < 0.0f ? 2 : 0) | (od1[k].y < 0.0f ? 4 : 0);
+			          : octantOf(dir[k].x, dir[k].y, dir[k].z);
 			for (int q = 0; q < kBuckets; q++) {
 				const unsigned long long m = __ballot(alive[k] && bucket[k] == q);
 				if (q == bucket[k]) mask[k] = m;
@@ -446,8 +540,8 @@ __global__ __launch_bounds__(kBlock) void wfShade(WfParams P, int cur) {
 			if (alive[k]) {
 				const uint32_t slot = tileBase + part[(bucket[k] * kShadePer + k) * kWaves + wave] +
 				                      (uint32_t) __popcll(mask[k] & below);
-				D.od0[slot] = od0[k];
-				D.od1[slot] = od1[k];
+				D.ori[slot] = ori[k];
+				D.dir[slot] = dir[k];
 				D.tl[slot] = tl[k];
 				D.meta[slot] = meta[k];
 			}
@@ -589,8 +683,8 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 		int size = 0;
 		bool overflow = false, trav = false;
 		if (valid) {
-			const float4 a = Q.od0[i], b = Q.od1[i];
-			ray = makeRay(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), Rp->fastSlabs != 0);
+			const float4 a = Q.ori[i], b = Q.dir[i];
+			ray = makeRay(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), Rp->fastSlabs != 0);
 			c.add(0, 1);
 			const SceneRange &R = *Rp;
 			if (R.useBvh == 0) {
@@ -777,13 +871,21 @@ using TraceFn = void (*)(WfParams, int);
 inline int stackStorage(const mrad_wavefront_config &c) {
 	return c.stack_storage > 0 ? c.stack_storage : (c.bvh_stack_size <= 64 ? 64 : 256);
 }
+// Shading fused into the trace kernel (wfTrace's kFuse) instead of the
+// separate wfShade pass; the regrouped trace keeps the separate pass.
+#ifndef MRAD_FUSED_SHADE
+#define MRAD_FUSED_SHADE 0
+#endif
+constexpr bool kFusedShade = MRAD_FUSED_SHADE != 0;
+inline bool fusedShade(const mrad_wavefront_config &c) { return kFusedShade && c.leaf_regroup == 0; }
 template <bool M>
 TraceFn traceFor(int storage, int ldsEntries, bool regroup) {
+	constexpr bool F = kFusedShade;
 	if (regroup) return storage <= 64 ? wfTraceRegroup<M, 64, 8> : wfTraceRegroup<M, 256, 8>;
 	if (ldsEntries <= 4)
-		return storage <= 32 ? wfTrace<M, 32, 4> : storage <= 64 ? wfTrace<M, 64, 4> : wfTrace<M, 256, 4>;
-	return storage <= 32 ? wfTrace<M, 32, kShortStack>
-	     : storage <= 64 ? wfTrace<M, 64, kShortStack> : wfTrace<M, 256, kShortStack>;
+		return storage <= 32 ? wfTrace<M, 32, 4, F> : storage <= 64 ? wfTrace<M, 64, 4, F> : wfTrace<M, 256, 4, F>;
+	return storage <= 32 ? wfTrace<M, 32, kShortStack, F>
+	     : storage <= 64 ? wfTrace<M, 64, kShortStack, F> : wfTrace<M, 256, kShortStack, F>;
 }
 
 }  // namespace mrad
@@ -891,8 +993,8 @@ WfParams params(mrad_batch *b) {
 	P.scenes = b->sceneTable.as<const WfScene>();
 	P.sceneCount = (int) b->scenes.size();
 	for (int q = 0; q < 2; q++) {
-		P.pool[q].od0 = b->pool[q][0].as<float4>();
-		P.pool[q].od1 = b->pool[q][1].as<float4>();
+		P.pool[q].ori = b->pool[q][0].as<float4>();
+		P.pool[q].dir = b->pool[q][1].as<float4>();
 		P.pool[q].tl = b->pool[q][2].as<float4>();
 		P.pool[q].meta = b->pool[q][3].as<uint4>();
 	}
@@ -909,8 +1011,9 @@ WfParams params(mrad_batch *b) {
 	return P;
 }
 
-// One bounce step on pool `cur`: generate, trace, shade into cur ^ 1.  With
-// `ev` (4 events) the kernel boundaries are recorded for per-kernel timing.
+// One bounce step on pool `cur`: generate, trace, shade into cur ^ 1 (the
+// shading inside the trace kernel when it is fused).  With `ev` (4 events) the
+// kernel boundaries are recorded for per-kernel timing.
 int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = nullptr) {
 	const int persistentBlocks = std::max(1, b->cuCount * b->cfg.waves_per_cu / kWaves);
 	// shading takes fixed tiles: one resident wave of blocks, whatever the
@@ -927,7 +1030,7 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 	                          : traceFor<false>(stackStorage(b->cfg), b->cfg.lds_stack_entries, regroup);
 	hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
-	hipLaunchKernelGGL(wfShade, dim3(shadeBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	if (!fusedShade(b->cfg)) hipLaunchKernelGGL(wfShade, dim3(shadeBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	hipLaunchKernelGGL(wfAdvanceShade, dim3(1), dim3(64), 0, b->stream, P, cur);
 	if (ev) MRAD_HIP(hipEventRecord(ev[3], b->stream));
 	MRAD_HIP(hipGetLastError());
