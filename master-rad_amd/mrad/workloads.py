@@ -180,6 +180,19 @@ def gpu_camera(camera: Camera, width: int, height: int) -> GpuCamera:
     return GpuCamera(cp, fwd, right, up)
 
 
+def primary_grid(camera: Camera, width: int, height: int, phase_x: float = 0.5, phase_y: float = 0.5) -> np.ndarray:
+    """BenchmarkRays.primaryGrid (:14-26): one ray per grid cell (x + phase_x, y + phase_y) of a width x height
+    sensor, rows from the top, as float32 (origin, direction) records."""
+    out = np.empty((width * height, 6), np.float32)
+    k = 0
+    for y in range(height):
+        for x in range(width):
+            o, d = camera.ray(*_sensor(x + phase_x, y + phase_y, width, height))
+            out[k] = (*o, *d)
+            k += 1
+    return out
+
+
 # ---- Layered Hero (graphics3d/scene/v1/LayeredHeroFamily.java)
 PRIMITIVES_PER_MODULE = 8
 _DEPTH_SEPARATION, _BOX_HALF_LENGTH, _BOX_DEPTH_TILT_TURNS = 0.750, 2.000, 0.150
@@ -401,6 +414,256 @@ def gi_test_camera() -> Camera:
 
 def gi_test_gpu_camera(width: int = GI_TEST_WIDTH, height: int = GI_TEST_HEIGHT) -> GpuCamera:
     return gpu_camera(gi_test_camera(), width, height)
+
+
+# ---- City of Night (graphics3d/scene/v1/CityOfNightV1.java) and Signal Chamber (SignalChamber.java), the
+# catalog's other introductory scenes (playground/SceneCatalog.java:39-46)
+_CITY_SEED = 0x1F234AB9C6D8EE07
+_CITY_EPS, _CITY_PANEL = 0.0025, 0.004
+_CITY_MOON = 4.0
+_CITY_WINDOWS = [(30.0, 14.0, 6.0), (10.0, 22.0, 40.0), (24.0, 12.0, 30.0), (16.0, 28.0, 18.0)]
+
+
+def _mul(c, k):
+    return (c[0] * k, c[1] * k, c[2] * k)
+
+
+def _clamp(v, lo, hi):
+    return max(lo, min(hi, v))
+
+
+class _City:
+    """CityOfNightV1's construction (:29-81, :83-389) in the reference's draw order.  Boxes are
+    SceneSupport.box(centre, radii) = axis-aligned [centre - radii, centre + radii]."""
+
+    def __init__(self):
+        from .random_scenes import SplittableRandom
+        self.r = SplittableRandom(_CITY_SEED)
+        self.boxes, self.mats = [], []
+
+    def box(self, c, radii, mat):
+        self.boxes.append((c[0] - radii[0], c[1] - radii[1], c[2] - radii[2],
+                           c[0] + radii[0], c[1] + radii[1], c[2] + radii[2]))
+        self.mats.append(mat)
+
+    def light(self, c, radii, glow):
+        self.box(c, radii, _surface(glow))
+
+    def front_back(self, x, y, z, hw, hh, rz, ht, glow, front):
+        zc = z + rz + ht + _CITY_EPS if front else z - rz - ht - _CITY_EPS
+        self.light((x, y, zc), (hw, hh, ht), glow)
+
+    def side(self, x, y, z, rx, ht, hh, hd, glow, right):
+        xc = x + rx + ht + _CITY_EPS if right else x - rx - ht - _CITY_EPS
+        self.light((xc, y, z), (ht, hh, hd), glow)
+
+    def build(self):
+        r = self.r
+        for gz in range(32):
+            z_base = -4.45 + gz * 0.60
+            for gx in range(-16, 17):
+                keep = 0.67 if gz < 6 else (0.63 if gz < 12 else 0.57)
+                if abs(gx) > 13:
+                    keep -= 0.08
+                if r.next_double() > keep:
+                    continue
+                x = gx * 0.64 + r.next_double_in(-0.10, 0.10)
+                z = z_base + r.next_double_in(-0.10, 0.10)
+                rx = r.next_double_in(0.10, 0.20)
+                rz = r.next_double_in(0.10, 0.20)
+                height = 0.10 + math.pow(r.next_double(), 1.5) * 0.18 + math.pow(r.next_double(), 4.2) * 1.10
+                if gz > 12 and abs(gx) < 8 and r.next_double() < 0.065:
+                    height *= r.next_double_in(2.2, 3.5)
+                    rx *= r.next_double_in(0.76, 0.92)
+                    rz *= r.next_double_in(0.76, 0.92)
+                self.tower(x, z, rx, rz, height, gz)
+        for _ in range(26):                                            # addForegroundMassing (:83-106)
+            x = r.next_double_in(-6.8, 6.8)
+            z = r.next_double_in(-5.4, -0.9)
+            rx = r.next_double_in(0.14, 0.32)
+            rz = r.next_double_in(0.16, 0.34)
+            height = r.next_double_in(0.12, 0.34)
+            diffuse = _hsb(r.next_double(), r.next_double_in(0.12, 0.32), r.next_double_in(0.16, 0.28))
+            self.box((x, height * 0.5, z), (rx, height * 0.5, rz),
+                     _surface((0.0, 0.0, 0.0), diffuse, (0.006, 0.007, 0.008)))
+        self.box((0.0, 5.55, 5.1), (13.5, 0.045, 15.0), _surface(_mul((0.22, 0.24, 0.27), _CITY_MOON)))
+        return self
+
+    def tower(self, x, z, rx, rz, height, band):                       # addTower (:108-152)
+        r = self.r
+        y = height * 0.5
+        mirror = r.next_double() < 0.04
+        colorful = r.next_double() < 0.62
+        if colorful:
+            diffuse = _hsb(r.next_double(), r.next_double_in(0.42, 0.82), r.next_double_in(0.34, 0.60))
+        else:
+            diffuse = _hsb(r.next_double(), r.next_double_in(0.16, 0.32), r.next_double_in(0.22, 0.36))
+        reflective = (0.44, 0.48, 0.54) if mirror else (0.026, 0.030, 0.034)
+        self.box((x, y, z), (rx, y, rz), _surface((0.0, 0.0, 0.0), diffuse, reflective))
+        if height > 0.78 and r.next_double() < 0.38:
+            self.box((x, 0.08, z), (rx * 1.08, 0.08, rz * 1.08),
+                     _surface((0.0, 0.0, 0.0), _mul(diffuse, 0.90), _mul(reflective, 0.55)))
+        glow = _CITY_WINDOWS[(r.next_int_bounded(len(_CITY_WINDOWS)) + band) % len(_CITY_WINDOWS)]
+        self.accents(x, z, rx, rz, height, glow)
+        pick = r.next_double()
+        if pick < 0.20:
+            self.dense_bands(x, z, rx, rz, height, glow)
+        elif pick < 0.32:
+            self.grid(x, z, rx, rz, height, glow, 0.64)
+        elif pick < 0.42:
+            self.grid(x, z, rx, rz, height, _mul(glow, 0.98), 0.40)
+        elif pick < 0.52:
+            self.corners(x, z, rx, rz, height, glow)
+
+    def accents(self, x, z, rx, rz, height, glow):                     # addFacadeAccents (:154-189)
+        r = self.r
+        if height > 0.48 and r.next_double() < 0.42:
+            self.strip_fb(x, z, rx, rz, height, True, _mul(glow, r.next_double_in(1.20, 1.66)))
+        if height > 0.54 and r.next_double() < 0.18:
+            self.strip_fb(x, z, rx, rz, height, False, _mul(glow, r.next_double_in(1.00, 1.30)))
+        if height > 0.48 and r.next_double() < 0.30:
+            self.strip_side(x, z, rx, rz, height, True, _mul(glow, r.next_double_in(1.12, 1.54)))
+        if height > 0.54 and r.next_double() < 0.18:
+            self.strip_side(x, z, rx, rz, height, False, _mul(glow, r.next_double_in(1.00, 1.34)))
+        if r.next_double() < 0.22:
+            self.billboard(x, z, rx, rz, height, True, _mul(glow, r.next_double_in(1.14, 1.44)))
+        if r.next_double() < 0.06:
+            h = height * r.next_double_in(0.72, 0.96)
+            self.billboard(x, z, rx, rz, h, False, _mul(glow, r.next_double_in(1.00, 1.18)))
+        if r.next_double() < 0.12:
+            self.side_billboard(x, z, rx, rz, height, True, _mul(glow, r.next_double_in(1.08, 1.42)))
+        if r.next_double() < 0.10:
+            self.side_billboard(x, z, rx, rz, height, False, _mul(glow, r.next_double_in(1.00, 1.28)))
+
+    def strip_fb(self, x, z, rx, rz, height, front, glow):            # addFrontBackStrip (:191-204)
+        hw = min(rx * 0.08, 0.010)
+        hh = min(height * 0.38, max(0.15, height * 0.28))
+        cy = _clamp(height * self.r.next_double_in(0.42, 0.62), hh + 0.04, height - hh - 0.04)
+        self.front_back(x, cy, z, hw, hh, rz, _CITY_PANEL, glow, front)
+
+    def strip_side(self, x, z, rx, rz, height, right, glow):          # addSideStrip (:206-220)
+        hd = min(rz * 0.08, 0.010)
+        hh = min(height * 0.36, max(0.15, height * 0.26))
+        cy = _clamp(height * self.r.next_double_in(0.40, 0.60), hh + 0.04, height - hh - 0.04)
+        self.side(x, cy, z, rx, _CITY_PANEL, hh, hd, glow, right)
+
+    def side_billboard(self, x, z, rx, rz, height, right, glow):      # addSideBillboard (:222-236)
+        r = self.r
+        hh = min(height * 0.24, 0.14 + r.next_double() * 0.08)
+        cy = _clamp(height * r.next_double_in(0.32, 0.72), hh + 0.05, height - hh - 0.05)
+        hd = min(rz * 0.70, 0.24 + r.next_double() * 0.10)
+        self.side(x, cy, z, rx, _CITY_PANEL, hh, hd, glow, right)
+
+    def dense_bands(self, x, z, rx, rz, height, glow):                # addDenseBands (:238-254)
+        r = self.r
+        bands = max(3, int(math.floor(height / 0.26)))
+        for i in range(bands):
+            y = 0.08 + i * (height - 0.16) / bands
+            if r.next_double() < 0.08:
+                continue
+            ws = 0.42 + r.next_double() * 0.34
+            g = _mul(glow, r.next_double_in(1.16, 1.52))
+            self.front_back(x, y, z, rx * ws, 0.016, rz, 0.010, g, True)           # addFrontBackWindow (:353-356)
+            self.front_back(x, y, z, rx * ws, 0.016, rz, 0.010, _mul(g, 0.86), False)
+
+    def grid(self, x, z, rx, rz, height, glow, lit_chance):            # addGrid (:278-303)
+        r = self.r
+        rows = max(4, int(math.floor(height / 0.26)))
+        cols = max(2, int(math.floor((rx * 2.0) / 0.11)))
+        hw = min(0.040, rx * 0.24)
+        for row in range(rows):
+            y = 0.10 + row * (height - 0.20) / rows
+            for col in range(cols):
+                if r.next_double() > lit_chance:
+                    continue
+                xo = 0.0 if cols == 1 else (-rx * 0.68 + col * (rx * 1.36 / (cols - 1)))
+                lit = _mul(glow, r.next_double_in(1.12, 1.64))
+                self.front_back(x + xo, y, z, hw, 0.015, rz, 0.010, lit, True)
+                if r.next_double() < 0.64:
+                    self.front_back(x + xo, y, z, hw, 0.015, rz, 0.010, _mul(lit, 0.88), False)
+
+    def corners(self, x, z, rx, rz, height, glow):                    # addCornerPattern (:305-321)
+        r = self.r
+        edge = height * 0.30
+        y = height * 0.50
+        lit = _mul(glow, r.next_double_in(1.12, 1.48))
+        self.front_back(x - rx * 0.72, y, z, 0.014, edge, rz, 0.010, lit, True)
+        self.front_back(x + rx * 0.72, y, z, 0.014, edge, rz, 0.010, _mul(lit, 0.94), True)
+        if r.next_double() < 0.54:
+            self.side(x, y, z - rz * 0.52, rx, 0.010, edge * 0.78, 0.015, _mul(lit, 0.84), False)
+
+    def billboard(self, x, z, rx, rz, height, front, glow):           # addBillboard (:323-337)
+        r = self.r
+        hh = min(height * 0.22, 0.14 + r.next_double() * 0.08)
+        cy = _clamp(height * r.next_double_in(0.30, 0.70), hh + 0.05, height - hh - 0.05)
+        hw = min(rx * 0.76, 0.28 + r.next_double() * 0.12)
+        lit = _mul(glow, r.next_double_in(1.24, 1.70))
+        self.front_back(x, cy, z, hw, hh, rz, _CITY_PANEL, lit, front)
+
+
+def city_of_night_scene(config: BvhBuildConfig | None = None) -> GpuScene:
+    """CityOfNightV1: a ground half-space (a dark glossy surface), towers of axis-aligned boxes on a jittered grid
+    with emissive window panels, strips and billboards, 26 foreground blocks and an emissive moon slab, all drawn
+    from SplittableRandom(0x1F234AB9C6D8EE07); background (0.021, 0.023, 0.031)."""
+    c = _City().build()
+    sc = GpuScene(config or BvhBuildConfig.uniform(LEAF_SIZE))
+    sc.set_background((0.021, 0.023, 0.031))
+    sc.add_boxes(np.array(c.boxes, np.float64), np.array(c.mats))
+    ground = _surface((0.0, 0.0, 0.0), (0.150, 0.154, 0.160), (0.024, 0.026, 0.028))
+    sc.add_planes([[0.0, 0.0, 0.0, 0.0, 1.0, 0.0]], np.array([ground]))
+    return sc
+
+
+def city_of_night_camera() -> Camera:
+    """SceneCatalog.create(CITY_OF_NIGHT_V1) (:39-42): SceneSupport.camera((0, 0.34, -6.6), 0.08, -0.01)."""
+    return support_camera((0.0, 0.34, -6.6), 0.08, -0.01)
+
+
+_SIGNAL_SEED = 0x5C82D0A4B7319E6F
+
+
+def _signal_color(r):
+    """SignalChamber.randomPixelColor (:91-101)."""
+    if r.next_double() < 0.08:
+        hot = r.next_double_in(10.0, 18.0)
+        return (hot, hot * r.next_double_in(0.95, 1.05), hot * r.next_double_in(1.00, 1.12))
+    base = _hsb(r.next_double(), r.next_double_in(0.55, 0.92), r.next_double_in(0.60, 1.00))
+    return _mul(base, r.next_double_in(9.0, 24.0))
+
+
+def signal_chamber_scene(config: BvhBuildConfig | None = None) -> GpuScene:
+    """SignalChamber: six half-spaces (reflective floor, ceiling, front and back, matte side walls) and two walls of
+    101 x 151 emissive spheres of radius 0.014 (30,502), colours from SplittableRandom(0x5C82D0A4B7319E6F);
+    black background."""
+    from .random_scenes import SplittableRandom
+    refl = _surface((0.0, 0.0, 0.0), (0.055, 0.060, 0.070), (0.44, 0.47, 0.52))
+    side = _surface((0.0, 0.0, 0.0), (0.05, 0.055, 0.065))
+    planes = [[0.0, -1.0, 0.0, 0.0, 1.0, 0.0], [0.0, 1.96, 0.0, 0.0, -1.0, 0.0], [-1.26, 0.0, 0.0, 1.0, 0.0, 0.0],
+              [1.26, 0.0, 0.0, -1.0, 0.0, 0.0], [0.0, 0.0, 6.50, 0.0, 0.0, -1.0], [0.0, 0.0, -2.60, 0.0, 0.0, 1.0]]
+    pmats = [refl, refl, side, side, refl, refl]
+    rows, cols = 101, 151
+    y0, z0 = -0.80, -0.10
+    ys, zs = 2.60 / (rows - 1), 6.35 / (cols - 1)
+    r = SplittableRandom(_SIGNAL_SEED)
+    spheres, mats = [], []
+    for row in range(rows):
+        for col in range(cols):
+            left, right = _signal_color(r), _signal_color(r)
+            y, z = y0 + row * ys, z0 + col * zs
+            spheres.append((-1.268, y, z, 0.014))
+            mats.append(_surface(left))
+            spheres.append((1.268, y, z, 0.014))
+            mats.append(_surface(right))
+    sc = GpuScene(config or BvhBuildConfig.uniform(LEAF_SIZE))
+    sc.set_background((0.0, 0.0, 0.0))
+    sc.add_spheres(np.array(spheres, np.float64), np.array(mats))
+    sc.add_planes(planes, np.array(pmats))
+    return sc
+
+
+def signal_chamber_camera() -> Camera:
+    """SceneCatalog.create(SIGNAL_CHAMBER) (:43-46): SceneSupport.camera((0, 0.20, -1.40), 0.016, 0)."""
+    return support_camera((0.0, 0.20, -1.40), 0.016, 0.0)
 
 
 def scale_id(scale: int) -> str:

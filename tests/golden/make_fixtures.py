@@ -3,6 +3,7 @@ result files (data only; run in a container where the reference is mounted).
 
   gi_test_topology.json  <- benchmarks/results/main-bvh-study/leaf-size-time.csv
                             (GI_TEST rows: leaf size, topology SHA-256, nodes, depth)
+  catalog_topology.json  <- the same file's CITY_OF_NIGHT_V1 and SIGNAL_CHAMBER rows
   hero_topology.json     <- benchmarks/results/main-bvh-study/hero-sensitivity.csv
                             (Layered Hero: scale, leaf size, lambda, topology SHA-256)
                             and core8-detail.csv (uniform / weighted rows at leaf 8)
@@ -40,6 +41,17 @@ def gi_test_topology():
                 rows.append({"leafSize": int(r["leafSize"]), "topologySha256": r["topologySha256"],
                              "nodeCount": int(r["nodeCount"]), "maxDepth": int(r["maxDepth"])})
     (OUT / "gi_test_topology.json").write_text(json.dumps(rows, indent=1) + "\n")
+
+
+def catalog_topology():
+    rows = []
+    with open(STUDY / "leaf-size-time.csv") as f:
+        for r in csv.DictReader(f):
+            if r["sceneId"] in ("CITY_OF_NIGHT_V1", "SIGNAL_CHAMBER"):
+                rows.append({"sceneId": r["sceneId"], "leafSize": int(r["leafSize"]),
+                             "topologySha256": r["topologySha256"], "nodeCount": int(r["nodeCount"]),
+                             "maxDepth": int(r["maxDepth"])})
+    (OUT / "catalog_topology.json").write_text(json.dumps(rows, indent=1) + "\n")
 
 
 def _core8_rows(ids):
@@ -125,6 +137,7 @@ def random_reductions():
 
 if __name__ == "__main__":
     gi_test_topology()
+    catalog_topology()
     hero_topology()
     hero_counters()
     overlap_rows()

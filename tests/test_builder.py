@@ -30,6 +30,23 @@ def test_gi_test_topology_hashes_match_published():
         assert st.node_count == row["nodeCount"] and st.max_depth == row["maxDepth"]
 
 
+@pytest.mark.parametrize("scene_id", ["CITY_OF_NIGHT_V1", "SIGNAL_CHAMBER"])
+def test_catalog_topology_hashes_match_published(scene_id):
+    """leaf-size-time.csv's City of Night and Signal Chamber rows (LeafSizeStudy.java:127-140: object SAH with the
+    calibrated base at lambda 0, which is the uniform cost): every leaf size's topology SHA-256, node count and
+    depth, under both the uniform and the lambda-0 weighted build."""
+    from mrad import workloads as W
+    make = W.city_of_night_scene if scene_id == "CITY_OF_NIGHT_V1" else W.signal_chamber_scene
+    rows = [r for r in json.loads((GOLDEN / "catalog_topology.json").read_text()) if r["sceneId"] == scene_id]
+    assert len(rows) == 7
+    for row in rows:
+        for cfg in (mrad.BvhBuildConfig.uniform(row["leafSize"]), mrad.BvhBuildConfig.weighted(row["leafSize"], 0.0)):
+            sc = make(cfg)
+            st = sc.bvh_stats()
+            assert topology_sha256(sc) == row["topologySha256"], (row, cfg.mode)
+            assert st.node_count == row["nodeCount"] and st.max_depth == row["maxDepth"], row
+
+
 def _random_small(rng, n_s, n_b, n_as, n_ab):
     spheres = np.concatenate([rng.uniform(-1, 1, (n_s, 3)), rng.uniform(0.02, 0.3, (n_s, 1))], 1)
     c = rng.uniform(-1, 1, (n_b, 3))
