@@ -136,6 +136,10 @@ typedef struct {
 	int pixels_per_launch;    /* gfxlab.gpu.renderPixelsPerLaunch (GpuLaunchProvenance.java:44-60): 0 = the
 	                             whole image per launch; else tiles of min(W, n) x max(1, n / that) pixels,
 	                             1..262144 */
+	int replay_rays_per_launch; /* gfxlab.gpu.replayRaysPerLaunch (GpuLaunchProvenance.java:30-33): rays per
+	                               physical replay launch, 1..262144 (0 = 262144); a linear replay also keeps
+	                               each launch within 2^20 primitive-work units per ray and 8192 launches
+	                               (GpuRayTracer.java:743-778) */
 	int64_t max_physical_kernel_nanos;  /* 0 = unchecked; else a frame with a physical launch at or above
 	                                       this fails with MRAD_ERR_ILLEGAL_STATE (the reference's 250 ms
 	                                       watchdog guard, GpuRayTracer.java:572-580) */
@@ -190,7 +194,10 @@ int mrad_copy_frame(mrad_tracer *tracer, float *rgb_out);
 /* GpuRayTracer.traceReplay(scene, rays, n, metrics) (GpuRayTracer.java:593-716)
  * for the resident scene; outputs as TraceReplayResult (:119-126).
  * per_ray_counters (n x 14 u32, slot order as MRAD_STAT_*) may be NULL;
- * it requires a tracer created with collect_metrics. */
+ * it requires a tracer created with collect_metrics.  The rays run in
+ * physical launches of at most replay_rays_per_launch; a linear tracer's plan
+ * (linearReplayRaysPerLaunch, :743-769) that exceeds the per-ray work budget
+ * or 8192 launches is MRAD_ERR_INVALID_ARGUMENT. */
 int mrad_trace_replay(mrad_tracer *tracer, const float *rays, int n, int collect_metrics,
                       int32_t *hit_flags, float *hit_distances, int32_t *hit_primitive_orders,
                       int32_t *hit_material_indices, float *hit_normals, uint32_t *per_ray_counters);

@@ -39,6 +39,13 @@ constexpr int kTile = 16;
 // MAX_RENDER_LAUNCHES_PER_CALL (GpuLaunchProvenance.java:5, GpuRayTracer.java:550-564).
 constexpr int kMaxPixelsPerLaunch = 262144;
 constexpr long long kMaxRenderLaunches = 8192;
+// The replay's launch plan (GpuRayTracer.java:64-76, :735-778): at most
+// 262144 rays per launch; a linear replay also bounds each launch's
+// primitive work (8 units per sphere, box or plane test, 64 per affine test)
+// to 2^20 units per ray and the call to 8192 launches.
+constexpr int kMaxReplayRaysPerLaunch = 262144;
+constexpr long long kMaxLinearReplayWork = 1LL << 20;
+constexpr long long kMaxLinearReplayChunks = 8192;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
 // XCD i mod 8).  Remapping i so that each XCD receives one contiguous range of
@@ -224,38 +231,89 @@ int replayState(mrad_tracer *t, uint32_t *perRay) {
 	return MRAD_OK;
 }
 
-// One replay launch on staging B: rays copied in, the kernel timed with the
-// handle's events, results copied out, frame statistics updated.  The
-// caller holds the handle's lock with its device current.
+int ensureEvents(mrad_tracer *t, int boundaries);
+
+// Rays per physical replay launch (launchTraceReplayLocked and
+// linearReplayRaysPerLaunch, GpuRayTracer.java:624-629, :743-778): the
+// configured limit, and for a linear replay the primitive-work budget per
+// launch and the chunk limit, with the reference's messages.
+int replayPlan(const mrad_tracer *t, int n, int &perLaunch) {
+	perLaunch = t->cfg.replay_rays_per_launch;
+	if (t->cfg.bvh_traversal) return MRAD_OK;
+	const SceneRange &R = t->range;
+	const long long work = 8LL * ((long long) R.sphCount + R.boxCount + R.planeCount) +
+	                       64LL * ((long long) R.asCount + R.abCount);
+	if (work > kMaxLinearReplayWork)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT,
+		                 "Linear replay requires " + std::to_string(work) +
+		                     " primitive-work units per ray, exceeding the watchdog-safe budget of " +
+		                     std::to_string(kMaxLinearReplayWork) + ". Use BVH traversal or a smaller scene.");
+	if (work > 0) perLaunch = (int) std::min<long long>(perLaunch, kMaxLinearReplayWork / work);
+	const long long chunks = ((long long) n + perLaunch - 1) / perLaunch;
+	if (chunks > kMaxLinearReplayChunks)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT,
+		                 "Linear replay would require " + std::to_string(chunks) +
+		                     " watchdog-safe chunks, exceeding the limit of " + std::to_string(kMaxLinearReplayChunks) +
+		                     ". Reduce the ray count or use BVH traversal.");
+	return MRAD_OK;
+}
+
+// A replay on staging B: rays copied in, launched in chunks of the replay
+// plan, the launches timed with the handle's events, results copied out,
+// frame statistics updated.  With launchEvents the boundary of every launch
+// is timed as well (events are created on first use); without, the longest
+// launch is reported as the whole interval.  The caller holds the handle's
+// lock with its device current.
 int replayWith(mrad_tracer *t, const ReplayBuffers &B, const float *rays, int n, int collect, int32_t *flags,
-               float *dist, int32_t *order, int32_t *mat, float *normals, uint32_t *perRay) {
+               float *dist, int32_t *order, int32_t *mat, float *normals, uint32_t *perRay, bool launchEvents) {
 	const int64_t t0 = nowNanos();
 	const size_t N = (size_t) n;
 	const bool metrics = collect && t->cfg.collect_metrics;
+	int perLaunch = 0;
+	MRAD_TRY(replayPlan(t, n, perLaunch));
+	const int launches = (int) (((long long) n + perLaunch - 1) / perLaunch);
+	if (launchEvents) MRAD_TRY(ensureEvents(t, launches - 1));
 	MRAD_HIP(hipMemcpyAsync(B.rays, rays, N * 24, hipMemcpyHostToDevice, t->stream));
 	MRAD_HIP(hipMemsetAsync(B.stats, 0, MRAD_STAT_COUNT * 8, t->stream));
 	ReplayParams P{};
 	P.S = t->arena.view;
 	P.R = t->range;
-	P.rays = B.rays;
-	P.n = n;
-	P.flags = B.flags;
-	P.dist = B.dist;
-	P.order = B.order;
-	P.mat = B.mat;
-	P.normals = B.normals;
-	P.perRay = perRay ? B.perRay : nullptr;
 	const bool instrumented = metrics || perRay;
 	P.stats = instrumented ? B.stats : nullptr;
 	P.matBase = 0;
 	ReplayFn fn = instrumented ? replayFor<true>(t->cfg.bvh_stack_size) : replayFor<false>(t->cfg.bvh_stack_size);
 	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
-	hipLaunchKernelGGL(fn, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, t->stream, P);
-	MRAD_HIP(hipGetLastError());
+	for (int l = 0; l < launches; l++) {
+		const size_t off = (size_t) l * (size_t) perLaunch;
+		const int m = (int) std::min<size_t>((size_t) perLaunch, N - off);
+		P.rays = B.rays + 6 * off;
+		P.n = m;
+		P.flags = B.flags + off;
+		P.dist = B.dist + off;
+		P.order = B.order + off;
+		P.mat = B.mat + off;
+		P.normals = B.normals + 3 * off;
+		P.perRay = perRay ? B.perRay + MRAD_STAT_COUNT * off : nullptr;
+		hipLaunchKernelGGL(fn, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0, t->stream, P);
+		MRAD_HIP(hipGetLastError());
+		if (launchEvents && l + 1 < launches) MRAD_HIP(hipEventRecord(t->boundary[l], t->stream));
+	}
 	MRAD_HIP(hipEventRecord(t->evEnd, t->stream));
 	MRAD_HIP(hipEventSynchronize(t->evEnd));
 	float ms = 0.0f;
 	MRAD_HIP(hipEventElapsedTime(&ms, t->evStart, t->evEnd));
+	int64_t maxPhys = (int64_t) (ms * 1.0e6f);
+	if (launchEvents && launches > 1) {
+		maxPhys = 0;
+		hipEvent_t prev = t->evStart;
+		for (int k = 0; k < launches; k++) {
+			hipEvent_t end = k + 1 == launches ? t->evEnd : t->boundary[k];
+			float pm = 0.0f;
+			MRAD_HIP(hipEventElapsedTime(&pm, prev, end));
+			maxPhys = std::max<int64_t>(maxPhys, (int64_t) (pm * 1.0e6f));
+			prev = end;
+		}
+	}
 	const int64_t c0 = nowNanos();
 	MRAD_HIP(hipMemcpy(flags, B.flags, N * 4, hipMemcpyDeviceToHost));
 	MRAD_HIP(hipMemcpy(dist, B.dist, N * 4, hipMemcpyDeviceToHost));
@@ -267,8 +325,8 @@ int replayWith(mrad_tracer *t, const ReplayBuffers &B, const float *rays, int n,
 	else std::memset(t->last.counters, 0, sizeof t->last.counters);
 	t->last.copy_nanos = nowNanos() - c0;
 	t->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
-	t->last.maximum_physical_kernel_nanos = t->last.kernel_nanos;
-	t->last.physical_launches = 1;
+	t->last.maximum_physical_kernel_nanos = std::min(maxPhys, t->last.kernel_nanos);
+	t->last.physical_launches = launches;
 	t->last.total_nanos = nowNanos() - t0;
 	return MRAD_OK;
 }
@@ -368,6 +426,9 @@ int mrad_create(const mrad_tracer_config *config, mrad_tracer **out) {
 	// GpuLaunchProvenance.renderTileLayout (:44-60): 1 <= pixels per launch <= 262144
 	if (c.pixels_per_launch < 0 || c.pixels_per_launch > kMaxPixelsPerLaunch)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "pixels per launch must be in [0, 262144]");
+	if (c.replay_rays_per_launch == 0) c.replay_rays_per_launch = kMaxReplayRaysPerLaunch;
+	if (c.replay_rays_per_launch < 1 || c.replay_rays_per_launch > kMaxReplayRaysPerLaunch)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "replay rays per launch must be in [0, 262144]");
 	if (c.max_physical_kernel_nanos < 0)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "maximum physical kernel nanos must be >= 0");
 	mrad_tracer *t = new (std::nothrow) mrad_tracer();
@@ -618,7 +679,7 @@ int mrad_trace_replay(mrad_tracer *t, const float *rays, int n, int collect, int
 	B.normals = t->normals.as<float>();
 	B.perRay = t->perRay.as<uint32_t>();
 	B.stats = t->stats.as<unsigned long long>();
-	return replayWith(t, B, rays, n, collect, flags, dist, order, mat, normals, perRay);
+	return replayWith(t, B, rays, n, collect, flags, dist, order, mat, normals, perRay, true);
 }
 
 // The replay of mrad_trace_replay with its device staging in the library's
@@ -645,7 +706,10 @@ int mrad_trace_replay_noalloc(mrad_tracer *t, const float *rays, int n, int coll
 	B.mat = reinterpret_cast<int *>(base + kScratchMat);
 	B.normals = reinterpret_cast<float *>(base + kScratchNormals);
 	B.perRay = reinterpret_cast<uint32_t *>(base + kScratchPerRay);
-	return replayWith(t, B, rays, n, collect, flags, dist, order, mat, normals, perRay);
+	// no per-launch events (creating them is a runtime allocation this entry
+	// point avoids): a multi-launch plan reports the whole interval as its
+	// longest launch
+	return replayWith(t, B, rays, n, collect, flags, dist, order, mat, normals, perRay, false);
 }
 
 int mrad_get_frame_stats(mrad_tracer *t, mrad_frame_stats *out) {

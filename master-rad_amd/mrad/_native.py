@@ -64,7 +64,7 @@ class TracerConfig(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("samples_per_frame", C.c_int),
                 ("bvh_traversal", C.c_int), ("collect_metrics", C.c_int), ("bvh_stack_size", C.c_int),
                 ("samples_per_launch", C.c_int), ("device", C.c_int), ("pixels_per_launch", C.c_int),
-                ("max_physical_kernel_nanos", C.c_int64)]
+                ("replay_rays_per_launch", C.c_int), ("max_physical_kernel_nanos", C.c_int64)]
 
 
 class FrameStats(C.Structure):

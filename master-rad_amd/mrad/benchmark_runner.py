@@ -198,6 +198,7 @@ def base_configuration(phase: str, process_id: int, width: int, height: int, max
 
 
 MAX_RENDER_PIXELS_PER_LAUNCH = 262_144      # GpuLaunchProvenance.java:5
+MAX_REPLAY_RAYS_PER_LAUNCH = 262_144        # GpuLaunchProvenance.java:6
 
 
 def render_tile_count(width: int, height: int, pixels_per_launch: int) -> int:
@@ -208,14 +209,15 @@ def render_tile_count(width: int, height: int, pixels_per_launch: int) -> int:
 
 
 def gpu_configuration(width: int, height: int, samples_per_frame: int,
-                      pixels_per_launch: int = MAX_RENDER_PIXELS_PER_LAUNCH) -> str:
+                      pixels_per_launch: int = MAX_RENDER_PIXELS_PER_LAUNCH,
+                      replay_rays_per_launch: int = MAX_REPLAY_RAYS_PER_LAUNCH) -> str:
     """The launch provenance appended for GPU versions (BenchmarkRunner.java:48-59).  As in the reference
     (GpuLaunchProvenance: SAMPLES_PER_PHYSICAL_KERNEL = 1, physicalRenderLaunches = tiles x samples), each
     physical launch renders one sample of one pixel tile of at most renderPixelsPerLaunch pixels; run()
     builds its renderers with samples_per_launch = 1 and that tile size to match."""
     launches = render_tile_count(width, height, pixels_per_launch) * samples_per_frame
     return (f",logicalSamplesPerFrame:{samples_per_frame},renderPixelsPerLaunch:{pixels_per_launch},"
-            f"replayRaysPerLaunch:{MAX_RENDER_PIXELS_PER_LAUNCH},samplesPerPhysicalKernel:{SAMPLES_PER_PHYSICAL_KERNEL},"
+            f"replayRaysPerLaunch:{replay_rays_per_launch},samplesPerPhysicalKernel:{SAMPLES_PER_PHYSICAL_KERNEL},"
             f"physicalKernelLaunchesPerLogicalFrame:{launches},kernelNanosSemantics:{KERNEL_NANOS_SEMANTICS}")
 
 
@@ -290,6 +292,9 @@ class Plan:
         self.pixels_per_launch = optional_int(props, "gfxlab.gpu.renderPixelsPerLaunch", MAX_RENDER_PIXELS_PER_LAUNCH)
         if not 1 <= self.pixels_per_launch <= MAX_RENDER_PIXELS_PER_LAUNCH:
             raise _invalid(f"gfxlab.gpu.renderPixelsPerLaunch must be between 1 and {MAX_RENDER_PIXELS_PER_LAUNCH}")
+        self.replay_rays_per_launch = optional_int(props, "gfxlab.gpu.replayRaysPerLaunch", MAX_REPLAY_RAYS_PER_LAUNCH)
+        if not 1 <= self.replay_rays_per_launch <= MAX_REPLAY_RAYS_PER_LAUNCH:
+            raise _invalid(f"gfxlab.gpu.replayRaysPerLaunch must be between 1 and {MAX_REPLAY_RAYS_PER_LAUNCH}")
         if not 8 <= self.stack_size <= 256:
             raise _invalid("gfxlab.gpu.bvhStackSize must be between 8 and 256")
         self.leaf_batch = optional_int(props, "gfxlab.gpu.leafBatch", 0)
@@ -302,7 +307,8 @@ class Plan:
         self.configs = {v: bvh_config(props, v, self.leaf_size) for v in self.versions}
 
     def configuration(self, scene_name: str = "") -> str:
-        text = self.base + gpu_configuration(self.width, self.height, self.samples_per_frame, self.pixels_per_launch)
+        text = self.base + gpu_configuration(self.width, self.height, self.samples_per_frame, self.pixels_per_launch,
+                                             self.replay_rays_per_launch)
         return text + f",layoutId:{self.layout_id}" if scene_name.startswith("f") else text
 
 

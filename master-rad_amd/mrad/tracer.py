@@ -108,12 +108,14 @@ class GpuRayTracer:
 
     def __init__(self, width: int, height: int, samples_per_frame: int, bvh_traversal: bool,
                  collect_metrics: bool = False, bvh_stack_size: int = 32, samples_per_launch: int = 0,
-                 device: int = 0, pixels_per_launch: int = 0, max_physical_kernel_nanos: int = 0):
-        """pixels_per_launch is gfxlab.gpu.renderPixelsPerLaunch (0: the whole image per launch) and
-        max_physical_kernel_nanos the per-launch duration guard (0: none; the reference uses 250 ms)."""
+                 device: int = 0, pixels_per_launch: int = 0, max_physical_kernel_nanos: int = 0,
+                 replay_rays_per_launch: int = 0):
+        """pixels_per_launch is gfxlab.gpu.renderPixelsPerLaunch (0: the whole image per launch),
+        replay_rays_per_launch gfxlab.gpu.replayRaysPerLaunch (0: 262144) and max_physical_kernel_nanos the
+        per-launch duration guard (0: none; the reference uses 250 ms)."""
         cfg = N.TracerConfig(int(width), int(height), int(samples_per_frame), 1 if bvh_traversal else 0,
                              1 if collect_metrics else 0, int(bvh_stack_size), int(samples_per_launch), int(device),
-                             int(pixels_per_launch), int(max_physical_kernel_nanos))
+                             int(pixels_per_launch), int(replay_rays_per_launch), int(max_physical_kernel_nanos))
         h = C.c_void_p()
         N.check(N.lib.mrad_create(C.byref(cfg), C.byref(h)))
         self._h = h

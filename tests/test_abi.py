@@ -185,6 +185,7 @@ def test_render_launch_plan_limits_are_checked_before_the_device():
         ranges = 1 if per == 0 else -(-spp // per)
         assert t.render_launch_count() == tiles * ranges, (w, h, spp, per, ppl)
         t.close()
-    for kw in (dict(pixels_per_launch=262_145), dict(pixels_per_launch=-1), dict(max_physical_kernel_nanos=-5)):
+    for kw in (dict(pixels_per_launch=262_145), dict(pixels_per_launch=-1), dict(max_physical_kernel_nanos=-5),
+               dict(replay_rays_per_launch=262_145), dict(replay_rays_per_launch=-1)):
         with pytest.raises(mrad.InvalidArgument):
             mrad.GpuRayTracer(8, 8, 1, True, **kw)

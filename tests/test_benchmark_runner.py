@@ -57,6 +57,8 @@ def test_plan_from_properties():
     ("gfxlab.gpu.samplesPerFrame", "0", "must be positive"),
     ("gfxlab.gpu.bvhStackSize", "4", "between 8 and 256"),
     ("gfxlab.gpu.leafBatch", "8", "0 or 16"),
+    ("gfxlab.gpu.replayRaysPerLaunch", "262145", "between 1 and 262144"),
+    ("gfxlab.gpu.renderPixelsPerLaunch", "0", "between 1 and 262144"),
     ("gfxlab.bvh.lambda", "-1", "nonnegative"),
     ("gfxlab.bvh.boxWeight", "0", "positive"),
     ("gfxlab.benchmarkDir", " ", "Missing required system property"),
@@ -140,3 +142,4 @@ def test_launch_plan_is_the_reference_tile_layout():
     assert B.render_tile_count(1920, 1080, 1000) == 2 * 1080
     assert B.render_tile_count(64, 32, 262_144) == 1
     assert "physicalKernelLaunchesPerLogicalFrame:16," in B.gpu_configuration(1920, 1080, 2)
+    assert ",replayRaysPerLaunch:4096," in B.gpu_configuration(1920, 1080, 2, replay_rays_per_launch=4096)

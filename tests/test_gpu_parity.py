@@ -56,6 +56,40 @@ def test_replay_bit_exact(require_gpu, name, use_bvh, stack):
         assert res.stats["stack_overflows"] > 0   # the capacity-8 run exercises the linear fallback
 
 
+def test_replay_launch_plan(require_gpu):
+    """launchTraceReplayLocked's chunks (GpuRayTracer.java:624-673): rays in launches of at most
+    replay_rays_per_launch give the results and counters of one launch; a linear replay's launches carry at most
+    2^20 primitive-work units per ray (8 per sphere, box or plane, 64 per affine primitive), and a plan beyond that
+    budget, or beyond 8192 launches, is an invalid argument (:743-769)."""
+    scene, cam = small_field(3000), configs.field_camera()
+    rays = _rays("field3k", cam)
+    ref = O.trace_replay(O.Scene(scene.wire()), rays)
+    with mrad.GpuRayTracer(8, 8, 1, True, collect_metrics=True, replay_rays_per_launch=1000) as t:
+        res = t.trace_replay(scene, rays, collect_metrics=True, per_ray_counters=True)
+        assert t.last_frame_stats()["physical_launches"] == -(-len(rays) // 1000)
+    assert_replay_equal(res, ref)
+    ns, nb, na = len(scene.sphere_data()), len(scene.box_data()), len(scene.affine_sphere_data())
+    work = 8 * (ns + nb + len(scene.plane_data())) + 64 * (na + len(scene.affine_box_data()))
+    per = (1 << 20) // work
+    lin = O.trace_replay(O.Scene(scene.wire(), use_bvh=False), rays)
+    with mrad.GpuRayTracer(8, 8, 1, False, collect_metrics=True) as t:
+        res = t.trace_replay(scene, rays, collect_metrics=True, per_ray_counters=True)
+        assert t.last_frame_stats()["physical_launches"] == -(-len(rays) // per)
+    assert_replay_equal(res, lin)
+    big = mrad.GpuScene()                      # 2^17 + 1 spheres: 8 (2^17 + 1) > 2^20 units per ray
+    big.add_spheres([[i * 1e-3, 0.0, 5.0, 1e-4] for i in range((1 << 17) + 1)],
+                    mrad.material_array((0.5, 0.5, 0.5), n=(1 << 17) + 1))
+    with mrad.GpuRayTracer(8, 8, 1, False) as t:
+        with pytest.raises(mrad.InvalidArgument, match="exceeding the watchdog-safe budget of 1048576"):
+            t.trace_replay(big, rays[:16])
+    edge = mrad.GpuScene()                     # exactly 2^20 units: one ray per launch, so 8193 rays need 8193
+    edge.add_spheres([[i * 1e-3, 0.0, 5.0, 1e-4] for i in range(1 << 17)], mrad.material_array((0.5, 0.5, 0.5), n=1 << 17))
+    many = np.tile(rays[:1], (8193, 1))
+    with mrad.GpuRayTracer(8, 8, 1, False) as t:
+        with pytest.raises(mrad.InvalidArgument, match="8193 watchdog-safe chunks, exceeding the limit of 8192"):
+            t.trace_replay(edge, many)
+
+
 def test_replay_uninstrumented_matches(require_gpu):
     scene, cam = small_field(3000), configs.field_camera()
     rays = _rays("field3k", cam)
