@@ -213,6 +213,8 @@ class GpuRayTracer:
         n = len(rays) // 6 if ray_count is None else int(ray_count)
         if n <= 0 or len(rays) < 6 * n:
             raise N.InvalidArgument(N.MRAD_ERR_INVALID_ARGUMENT, "Replay rays must contain origin.xyz and direction.xyz.")
+        if scene is None:                                               # requireReplayScene (:725-729)
+            raise N.InvalidArgument(N.MRAD_ERR_INVALID_ARGUMENT, "GPU scene must not be null")
         self.ensure_scene_resident(scene)
         flags = np.zeros(n, np.int32)
         dist = np.zeros(n, np.float32)

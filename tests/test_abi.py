@@ -77,6 +77,10 @@ def test_argument_checks_precede_device_checks():
         t.render_sample(np.zeros(10, np.float32), sc, cam, 4, 0, 1)
     with pytest.raises(mrad.InvalidArgument):
         t.trace_replay(sc, np.zeros(5, np.float32))
+    with pytest.raises(mrad.InvalidArgument, match="GPU scene must not be null"):
+        t.trace_replay(None, np.zeros(6, np.float32))
+    with pytest.raises(mrad.InvalidArgument, match="GPU scene and camera must not be null"):
+        t.render_sample(np.zeros(48, np.float32), None, cam, 4, 0, 1)
     rc = N.lib.mrad_upload_scene(None, None, 0)
     assert rc == N.MRAD_ERR_INVALID_ARGUMENT
     assert b"null" in N.lib.mrad_last_error()
