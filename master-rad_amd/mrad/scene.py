@@ -15,6 +15,11 @@ import numpy as np
 
 from . import _native as N
 
+# GpuScene.java:19-21: the largest affine spectral condition an upload accepts, and the smallest sphere radius or
+# box extent that survives float packing (sqrt of the smallest normal float).
+MAX_AFFINE_SPECTRAL_CONDITION = 65_536.0
+MIN_PACKED_PRIMITIVE_EXTENT = float(np.sqrt(np.float32(np.finfo(np.float32).tiny), dtype=np.float32))
+
 # PrimitiveCostModel.java:11-19 (RTX 3080 calibration of the reference).
 SPHERE = 1.000000000
 BOX = 1.288951704
@@ -201,6 +206,17 @@ class GpuScene:
     def primitive_count(self) -> int:
         c = self.counts()
         return c["spheres"] + c["boxes"] + c["planes"] + c["affine_spheres"] + c["affine_boxes"]
+
+    def material_count(self) -> int:
+        """materialCount (GpuScene.java:493-495): the distinct materials registered so far."""
+        return self.counts()["materials"]
+
+    def material_index_of(self, material) -> int:
+        """materialIndexOf (:497-500): the packed index of a material given as its 13 values in packed order
+        (diffuse, reflective, refractive, emittance, ior), or -1 if the scene has no such material."""
+        row = np.asarray(material, np.float64).reshape(13).astype(np.float32)
+        hits = np.flatnonzero(np.all(self.material_data() == row[None, :], axis=1))
+        return int(hits[0]) if len(hits) else -1
 
     def bvh_stats(self) -> N.BvhStats:
         s = N.BvhStats()

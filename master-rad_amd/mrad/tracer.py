@@ -146,6 +146,10 @@ class GpuRayTracer:
     def is_available(self) -> bool:
         return bool(N.lib.mrad_is_available(self._handle()))
 
+    def collects_metrics(self) -> bool:
+        """collectsMetrics (GpuRayTracer.java:289-292)."""
+        return self.collect_metrics
+
     def set_russian_roulette(self, start_depth: int) -> None:
         N.check(N.lib.mrad_set_russian_roulette(self._handle(), int(start_depth)))
 

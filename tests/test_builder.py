@@ -180,3 +180,24 @@ def test_affine_payload_matches_builder_rules():
     # sphere bounds: centre +- row lengths of the forward linear map (2, 1, 0.5), widened by 2 ulp
     np.testing.assert_allclose(nb, [0.3 - 2, -0.2 - 1, 0.1 - 0.5, 0.3 + 2, -0.2 + 1, 0.1 + 0.5], atol=1e-6)
     assert nb[0] < np.float32(0.3 - 2) and nb[3] > np.float32(0.3 + 2)
+
+
+def test_material_registry_and_packing_limits():
+    """materialCount / materialIndexOf (GpuScene.java:493-500): materials are registered once, in first-use order;
+    MIN_PACKED_PRIMITIVE_EXTENT (:20-21) is the smallest radius a sphere may keep after float packing."""
+    from mrad import scene as S
+    red, grey = mrad.material_array((0.8, 0.1, 0.1)), mrad.material_array((0.5, 0.5, 0.5))
+    sc = mrad.GpuScene()
+    sc.add_spheres([[0, 0, 0, 1], [2, 0, 0, 1], [4, 0, 0, 1]], np.concatenate([red, grey, red]))
+    assert sc.material_count() == 2
+    assert sc.material_index_of(red[0]) == 0 and sc.material_index_of(grey[0]) == 1
+    assert sc.material_index_of(mrad.material_array((0.1, 0.2, 0.3))[0]) == -1
+    assert S.MIN_PACKED_PRIMITIVE_EXTENT == float(np.float32(np.sqrt(np.float32(2.0 ** -126))))
+    sc.add_spheres([[0, 0, 0, S.MIN_PACKED_PRIMITIVE_EXTENT]], grey)       # at the origin its bounds stay apart
+    with pytest.raises(mrad.InvalidArgument, match="Sphere radius"):
+        sc.add_spheres([[0, 0, 0, S.MIN_PACKED_PRIMITIVE_EXTENT / 2]], grey)
+    with pytest.raises(mrad.InvalidArgument, match="Sphere bounds"):     # at y = 5 they collapse in float
+        sc.add_spheres([[0, 5, 0, S.MIN_PACKED_PRIMITIVE_EXTENT]], grey)
+    t = mrad.GpuRayTracer(4, 4, 1, True, collect_metrics=True)
+    assert t.collects_metrics()
+    t.close()
