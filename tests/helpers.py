@@ -90,3 +90,71 @@ def bad_material_variants(scene):
             arr[len(arr) // 2, col] = bad
             out.append((f"{name}={bad}", {name: arr}))
     return out
+
+
+class WireScene:
+    """A scene given by its wire arrays, as a caller that builds its own BVH would pass them to the boundary (for
+    example the per-type forests of the reference's PER_TYPE_SAH modes, up to four roots).  It provides what the
+    tracer and the oracle use of GpuScene: arrays(), key and wire()."""
+    _next_key = 1 << 40
+
+    def __init__(self, wire: dict):
+        import ctypes as C
+
+        from mrad import _native as N
+        self._wire = dict(wire)
+        self.key = WireScene._next_key
+        WireScene._next_key += 1
+        v = N.SceneArrays()
+        self._keep = []
+
+        def put(field, count_field, name, ptr, dtype, width):
+            a = np.ascontiguousarray(wire[name], dtype).reshape(-1)
+            self._keep.append(a)
+            setattr(v, field, a.ctypes.data_as(ptr))
+            if count_field:
+                setattr(v, count_field, len(a) // width)
+
+        put("spheres", "sphere_count", "spheres", N.fp, np.float32, 5)
+        put("boxes", "box_count", "boxes", N.fp, np.float32, 7)
+        put("planes", "plane_count", "planes", N.fp, np.float32, 26)
+        put("affine_spheres", "affine_sphere_count", "affine_spheres", N.fp, np.float32, 22)
+        put("affine_boxes", "affine_box_count", "affine_boxes", N.fp, np.float32, 22)
+        put("node_bounds", "node_count", "node_bounds", N.fp, np.float32, 6)
+        put("node_data", None, "node_data", N.ip, np.int32, 4)
+        put("primitive_refs", "primitive_ref_count", "refs", N.ip, np.int32, 1)
+        put("root_indices", "root_count", "roots", N.ip, np.int32, 1)
+        put("materials", "material_count", "materials", N.fp, np.float32, 13)
+        v.background = (C.c_float * 3)(*[float(x) for x in wire["background"]])
+        v.revision = 1
+        self._view = v
+
+    def arrays(self):
+        return self._view
+
+    def wire(self) -> dict:
+        return self._wire
+
+
+def per_type_forest(full, parts) -> WireScene:
+    """The wire arrays of `full` with its BVH replaced by a forest of one tree per primitive class: the trees of
+    `parts` (scenes holding one class each, built from the same primitives in the same order as `full`),
+    concatenated in order with their child and ref indices rebased; one root per tree."""
+    bounds, data, refs, roots = [], [], [], []
+    node_base = ref_base = 0
+    for p in parts:
+        nd = p.bvh_node_data().copy()
+        interior = nd[:, 0] >= 0
+        nd[interior, 0] += node_base
+        nd[interior, 1] += node_base
+        nd[~interior, 2] += ref_base
+        bounds.append(p.bvh_node_bounds())
+        data.append(nd)
+        refs.append(p.bvh_primitive_refs())
+        roots.extend(int(r) + node_base for r in p.bvh_root_indices())
+        node_base += len(nd)
+        ref_base += len(refs[-1])
+    wire = full.wire()
+    wire.update(node_bounds=np.concatenate(bounds), node_data=np.concatenate(data), refs=np.concatenate(refs),
+                roots=np.array(roots, np.int32))
+    return WireScene(wire)

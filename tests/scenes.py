@@ -56,3 +56,30 @@ def gate_methods():
 
 def gate_rays(camera):
     return W.primary_grid(camera, *GATE_GRID, 0.5, 0.5)
+
+
+def per_type_parts(seed: int = 17, n_sph: int = 300, n_box: int = 200, n_as: int = 60, n_ab: int = 100):
+    """A field of spheres, boxes, affine spheres and affine boxes as one scene (`full`, one BVH) and as one scene
+    per class (`parts`, same primitives in the same order), from which tests build per-type forests."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    mats = lambda n: mrad.material_array(rng.uniform(0.1, 0.9, (n, 3)), emittance=np.where(
+        rng.uniform(size=(n, 1)) < 0.03, rng.uniform(5, 20, (n, 1)), 0.0) * np.ones((1, 3)), n=n)
+    sph = np.concatenate([rng.uniform(-1, 1, (n_sph, 3)), rng.uniform(0.02, 0.08, (n_sph, 1))], 1)
+    c, e = rng.uniform(-1, 1, (n_box, 3)), rng.uniform(0.01, 0.06, (n_box, 3))
+    box = np.concatenate([c - e, c + e], 1)
+    tas = configs._transforms(configs._rotations(rng, n_as), rng.uniform(0.02, 0.08, (n_as, 3)),
+                              rng.uniform(-1, 1, (n_as, 3)))
+    tab = configs._transforms(configs._rotations(rng, n_ab), rng.uniform(0.02, 0.08, (n_ab, 3)),
+                              rng.uniform(-1, 1, (n_ab, 3)))
+    ms, mb, mas, mab = mats(n_sph), mats(n_box), mats(n_as), mats(n_ab)
+    cfg = mrad.BvhBuildConfig.weighted(4)
+    full = mrad.GpuScene(cfg)
+    full.add_spheres(sph, ms).add_boxes(box, mb).add_affine_transforms("sphere", tas, mas)
+    full.add_affine_transforms("box", tab, mab)
+    parts = [mrad.GpuScene(cfg).add_spheres(sph, ms), mrad.GpuScene(cfg).add_boxes(box, mb),
+             mrad.GpuScene(cfg).add_affine_transforms("sphere", tas, mas),
+             mrad.GpuScene(cfg).add_affine_transforms("box", tab, mab)]
+    for s in (full, *parts):
+        s.set_background((0.18, 0.22, 0.30))
+    return full, parts
