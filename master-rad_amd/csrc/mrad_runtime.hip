@@ -272,6 +272,9 @@ int replayWith(mrad_tracer *t, const ReplayBuffers &B, const float *rays, int n,
 	int perLaunch = 0;
 	MRAD_TRY(replayPlan(t, n, perLaunch));
 	const int launches = (int) (((long long) n + perLaunch - 1) / perLaunch);
+	// a BVH replay has no launch limit: beyond 8192 launches no boundary
+	// events are kept (one event per launch would grow without bound)
+	launchEvents = launchEvents && launches <= kMaxLinearReplayChunks;
 	if (launchEvents) MRAD_TRY(ensureEvents(t, launches - 1));
 	MRAD_HIP(hipMemcpyAsync(B.rays, rays, N * 24, hipMemcpyHostToDevice, t->stream));
 	MRAD_HIP(hipMemsetAsync(B.stats, 0, MRAD_STAT_COUNT * 8, t->stream));
