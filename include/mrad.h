@@ -175,6 +175,18 @@ int mrad_set_russian_roulette(mrad_tracer *tracer, int start_depth);
  * Layered Hero frames) render faster with 16, the seeded fields without.
  * No counterpart in the reference.  Other values: invalid argument. */
 int mrad_set_leaf_batch(mrad_tracer *tracer, int lanes);
+/* The form mrad_render_sample renders in.  MRAD_RENDER_FORM_MEGAKERNEL (the
+ * default) is the reference's per-pixel kernel under its launch contract
+ * (pixel tiles, sample ranges, the watchdog limit).  MRAD_RENDER_FORM_WAVEFRONT
+ * renders the same frame with the wavefront form's bounded path pool and
+ * persistent trace kernel over the tracer's resident scene: the image and the
+ * 14 counters are bit-identical, large frames render faster, the physical
+ * launches are its bounce steps (bounded by the pool, not by pixel tiles) and
+ * the per-launch watchdog limit is not checked.  No counterpart in the
+ * reference.  Other values: invalid argument. */
+#define MRAD_RENDER_FORM_MEGAKERNEL 0
+#define MRAD_RENDER_FORM_WAVEFRONT 1
+int mrad_set_render_form(mrad_tracer *tracer, int form);
 /* GpuRayTracer.renderSample(rgb, scene, camera, maxDepth, frameIndex,
  * frameSeed) (GpuRayTracer.java:317-432).  camera = position, forward,
  * right, up (GpuCamera.java:8-22).  rgb_out (W*H*3 floats) may be NULL to

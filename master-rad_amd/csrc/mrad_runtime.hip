@@ -470,6 +470,8 @@ void mrad_destroy(mrad_tracer *t) {
 			if (t->evEnd) (void) hipEventDestroy(t->evEnd);
 			if (t->stream) (void) hipStreamDestroy(t->stream);
 		}
+		mrad_batch_destroy(t->wf);
+		t->wf = nullptr;
 	}
 	delete t;
 }
@@ -492,6 +494,16 @@ int mrad_set_leaf_batch(mrad_tracer *t, int lanes) {
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "leaf batch must be 0 or " + std::to_string(kLeafBatchLanes));
 	std::lock_guard<std::mutex> g(t->mu);
 	t->leafBatch = lanes;
+	return MRAD_OK;
+}
+
+int mrad_set_render_form(mrad_tracer *t, int form) {
+	clear_error();
+	if (!t) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle is null");
+	if (form != MRAD_RENDER_FORM_MEGAKERNEL && form != MRAD_RENDER_FORM_WAVEFRONT)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "render form must be 0 (megakernel) or 1 (wavefront)");
+	std::lock_guard<std::mutex> g(t->mu);
+	t->renderForm = form;
 	return MRAD_OK;
 }
 
@@ -557,6 +569,31 @@ int mrad_render_launch_count(const mrad_tracer *t, int *launches) {
 	return MRAD_OK;
 }
 
+// renderSample in the wavefront form (mrad_set_render_form); the caller holds
+// the handle's lock with its device current and a scene resident.  The frame's
+// physical launches are its bounce steps; the longest one is not timed alone,
+// so the whole kernel interval is reported as the longest.
+static int renderWavefront(mrad_tracer *t, float *rgb, const float camera[12], int maxDepth, int frameIndex,
+                           uint64_t frameSeed) {
+	const int64_t frameStart = nowNanos();
+	const int W = t->cfg.width, H = t->cfg.height;
+	MRAD_TRY(ensure(t->output, (size_t) W * H * 3 * sizeof(float)));
+	MRAD_HIP(hipStreamSynchronize(t->stream));          // the resident arena, on the handle's stream
+	const WavefrontView v{camera, W, H, t->cfg.samples_per_frame, maxDepth, t->rrStart, frameIndex, frameSeed};
+	mrad_batch_stats st{};
+	MRAD_TRY(wavefrontFrame(&t->wf, t->cfg, t->arena.view, t->range, v, t->output.as<float>(), &st));
+	t->last.kernel_nanos = st.kernel_nanos;
+	t->last.maximum_physical_kernel_nanos = st.kernel_nanos;
+	t->last.physical_launches = (int) std::min<int64_t>(st.steps, 0x7FFFFFFF);
+	if (t->cfg.collect_metrics) std::memcpy(t->last.counters, st.counters, sizeof t->last.counters);
+	else std::memset(t->last.counters, 0, sizeof t->last.counters);
+	const int64_t copyStart = nowNanos();
+	if (rgb) MRAD_HIP(hipMemcpy(rgb, t->output.p, (size_t) W * H * 3 * sizeof(float), hipMemcpyDeviceToHost));
+	t->last.copy_nanos = rgb ? nowNanos() - copyStart : 0;
+	t->last.total_nanos = nowNanos() - frameStart;
+	return MRAD_OK;
+}
+
 // renderSample (GpuRayTracer.java:317-432).
 int mrad_render_sample(mrad_tracer *t, float *rgb, const float camera[12], int maxDepth, int frameIndex,
                        uint64_t frameSeed) {
@@ -566,11 +603,13 @@ int mrad_render_sample(mrad_tracer *t, float *rgb, const float camera[12], int m
 	if (!t) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle is null");
 	const int W = t->cfg.width, H = t->cfg.height, spp = t->cfg.samples_per_frame, per = t->cfg.samples_per_launch;
 	LaunchPlan L;
-	MRAD_TRY(launchPlan(t, L));                                     // before the device, as the reference orders it
+	if (t->renderForm == MRAD_RENDER_FORM_MEGAKERNEL)
+		MRAD_TRY(launchPlan(t, L));                                 // before the device, as the reference orders it
 	MRAD_TRY(requireTracer(t));
 	std::lock_guard<std::mutex> g(t->mu);
 	if (!t->hasScene) return set_error(MRAD_ERR_ILLEGAL_STATE, "no scene is resident: call mrad_upload_scene first");
 	MRAD_HIP(hipSetDevice(t->cfg.device));
+	if (t->renderForm == MRAD_RENDER_FORM_WAVEFRONT) return renderWavefront(t, rgb, camera, maxDepth, frameIndex, frameSeed);
 	const int64_t frameStart = nowNanos();
 	const int launches = (int) L.launches;
 	MRAD_TRY(ensure(t->output, (size_t) W * H * 3 * sizeof(float)));

@@ -27,9 +27,23 @@ struct mrad_tracer {
 	int rrStart = -1;
 	int leafBatch = 0;                  // megakernel leaf batching: 0 or kLeafBatchLanes
 	mrad_frame_stats last{};
+	int renderForm = MRAD_RENDER_FORM_MEGAKERNEL;
+	mrad_batch *wf = nullptr;           // the wavefront form's batch, created on first use
 };
 
 namespace mrad {
+
+// One frame of the resident scene in the wavefront form (mrad_wavefront.hip):
+// a batch of one view over the tracer's own device arena, created on first
+// use and kept for later frames; the image goes to `out` (W x H x 3 device
+// floats), bit-identical to the megakernel's.
+struct WavefrontView {
+	const float *cam;
+	int width, height, spp, maxDepth, rrStart, frameIndex;
+	uint64_t frameSeed;
+};
+int wavefrontFrame(mrad_batch **batch, const mrad_tracer_config &cfg, const DevScene &S, const SceneRange &R,
+                   const WavefrontView &v, float *out, mrad_batch_stats *stats);
 
 inline int requireTracer(mrad_tracer *t) {
 	if (!t) return set_error(MRAD_ERR_INVALID_ARGUMENT, "tracer handle is null");

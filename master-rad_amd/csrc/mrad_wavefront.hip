@@ -35,6 +35,7 @@
 
 #include "mrad_host.hpp"
 #include "mrad_kernels_common.hpp"
+#include "mrad_tracer.hpp"
 
 namespace mrad {
 
@@ -914,6 +915,10 @@ struct mrad_batch {
 	uint32_t chunkPaths = 0;             // result slots per buffer of the current plan
 	uint32_t *hostCounters = nullptr;    // pinned: counter block read back after each graph launch
 	mrad_batch_stats last{};
+	// a batch of one view over an arena owned elsewhere (the tracer's resident
+	// scene, wavefrontFrame): no arena upload of its own
+	bool external = false;
+	DevScene externalView{};
 };
 
 namespace {
@@ -966,7 +971,7 @@ void planChunks(mrad_batch *b) {
 int prepare(mrad_batch *b) {
 	if (!b->dirty) return MRAD_OK;
 	dropGraph(b);
-	MRAD_TRY(b->arena.upload(b->stream, b->host));
+	if (!b->external) MRAD_TRY(b->arena.upload(b->stream, b->host));
 	MRAD_TRY(upload(b->stream, b->sceneTable, b->scenes));
 	b->chunkPaths = chunkPathsFor(b->cfg, b->maxSpp);
 	planChunks(b);
@@ -989,7 +994,7 @@ int prepare(mrad_batch *b) {
 
 WfParams params(mrad_batch *b) {
 	WfParams P{};
-	P.S = b->arena.view;
+	P.S = b->external ? b->externalView : b->arena.view;
 	P.scenes = b->sceneTable.as<const WfScene>();
 	P.sceneCount = (int) b->scenes.size();
 	for (int q = 0; q < 2; q++) {
@@ -1091,6 +1096,91 @@ int buildGraph(mrad_batch *b, const WfParams &P) {
 		b->graph = nullptr;
 		return set_error(MRAD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
 	}
+	return MRAD_OK;
+}
+
+// The render of mrad_batch_render; the caller holds the batch's lock with its
+// device current and the batch holds at least one scene.
+int renderLocked(mrad_batch *b) {
+	const int64_t t0 = nowNanos();
+	const bool wasDirty = b->dirty;
+	MRAD_TRY(prepare(b));
+	b->last.upload_nanos = wasDirty ? nowNanos() - t0 : 0;
+	const WfParams P = params(b);
+	const bool timed = b->cfg.time_kernels != 0;
+	if (b->cfg.use_graph && !timed) MRAD_TRY(buildGraph(b, P));
+	b->last.trace_nanos = b->last.shade_nanos = b->last.generate_nanos = 0;
+	while (timed && b->stepEvents.size() < 4) {
+		hipEvent_t e;
+		MRAD_HIP(hipEventCreate(&e));
+		b->stepEvents.push_back(e);
+	}
+	// counters: zero, chunks 0 and 1 released (a kernel, not a memset or copy)
+	const uint32_t chunkCount = (uint32_t) b->chunks.size();
+	hipLaunchKernelGGL(wfInitCounters, dim3(1), dim3(64), 0, b->stream, P);
+	MRAD_HIP(hipGetLastError());
+	if (b->cfg.collect_metrics) MRAD_HIP(hipMemsetAsync(b->stats.p, 0, MRAD_STAT_COUNT * 8, b->stream));
+	MRAD_HIP(hipEventRecord(b->evStart, b->stream));
+	uint32_t resolved = 0;        // chunks resolved so far, in order
+	int64_t steps = 0;
+	const int64_t stepGuard = 64 + 4 * 33 * ((int64_t) (b->totalPaths / 64) + chunkCount + 1);
+	for (;;) {
+		if (timed) {
+			for (int cur = 0; cur < 2; cur++) {
+				MRAD_TRY(enqueueStep(b, P, cur, b->stepEvents.data()));
+				MRAD_TRY(accumulateStep(b, b->stepEvents.data()));
+			}
+		} else if (b->graph) {
+			MRAD_HIP(hipGraphLaunch(b->graph, b->stream));
+		} else {
+			MRAD_TRY(enqueueStep(b, P, 0));
+			MRAD_TRY(enqueueStep(b, P, 1));
+		}
+		steps += 2;
+		MRAD_HIP(hipMemcpyAsync(b->hostCounters, b->counters.p, kCounterWords * 4, hipMemcpyDeviceToHost, b->stream));
+		MRAD_HIP(hipStreamSynchronize(b->stream));
+		// Every chunk whose paths have all finished is resolved, in order; its
+		// buffer's count is reset and the chunk two ahead may then be generated
+		// into it.  Both kernels are stream-ordered before the next graph launch.
+		// Only chunks released before this snapshot can be complete: the count
+		// word of chunk r + 2 still holds chunk r's count until the release.
+		const uint32_t *c = b->hostCounters;
+		const uint32_t before = resolved;
+		while (resolved < chunkCount && resolved < c[kGenLimit] &&
+		       c[kDone0 + (resolved & 1u)] == b->chunks[resolved].pathCount) {
+			const uint32_t px = b->chunks[resolved].pixelCount;
+			const int blocks = std::max(1, std::min(16384, (int) ((px + kBlock - 1) / kBlock)));
+			hipLaunchKernelGGL(wfResolve, dim3(blocks), dim3(kBlock), 0, b->stream, P, resolved);
+			MRAD_HIP(hipGetLastError());
+			resolved++;
+		}
+		if (resolved == chunkCount) break;
+		if (resolved != before) {
+			hipLaunchKernelGGL(wfRelease, dim3(1), dim3(64), 0, b->stream, P, before, resolved);
+			MRAD_HIP(hipGetLastError());
+		}
+		if (steps > stepGuard) {
+			const uint32_t r = resolved;
+			return set_error(MRAD_ERR_ILLEGAL_STATE,
+			                 "wavefront did not drain: " + std::to_string(r) + " of " + std::to_string(chunkCount) +
+			                     " chunks resolved (chunk " + std::to_string(r) + ": " +
+			                     std::to_string(b->chunks[r].pathCount) + " paths, " +
+			                     std::to_string(c[kDone0 + (r & 1u)]) + " finished); pools " + std::to_string(c[kCount0]) +
+			                     " / " + std::to_string(c[kCount1]) + ", generating chunk " + std::to_string(c[kGenChunk]) +
+			                     " at " + std::to_string(c[kGenOffset]) + " (limit " + std::to_string(c[kGenLimit]) +
+			                     "), finished " + std::to_string(c[kDone0]) + " / " + std::to_string(c[kDone1]));
+		}
+	}
+	MRAD_HIP(hipEventRecord(b->evEnd, b->stream));
+	MRAD_HIP(hipEventSynchronize(b->evEnd));
+	float ms = 0.0f;
+	MRAD_HIP(hipEventElapsedTime(&ms, b->evStart, b->evEnd));
+	b->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
+	b->last.steps = steps;
+	b->last.paths = b->totalPaths;
+	if (b->cfg.collect_metrics) MRAD_HIP(hipMemcpy(b->last.counters, b->stats.p, sizeof b->last.counters, hipMemcpyDeviceToHost));
+	else std::memset(b->last.counters, 0, sizeof b->last.counters);
+	b->last.total_nanos = nowNanos() - t0;
 	return MRAD_OK;
 }
 
@@ -1196,87 +1286,9 @@ int mrad_batch_render(mrad_batch *b) {
 	std::lock_guard<std::mutex> g(b->mu);
 	if (b->scenes.empty()) return set_error(MRAD_ERR_ILLEGAL_STATE, "the batch holds no scene");
 	MRAD_HIP(hipSetDevice(b->cfg.device));
-	const int64_t t0 = nowNanos();
-	const bool wasDirty = b->dirty;
-	MRAD_TRY(prepare(b));
-	b->last.upload_nanos = wasDirty ? nowNanos() - t0 : 0;
-	const WfParams P = params(b);
-	const bool timed = b->cfg.time_kernels != 0;
-	if (b->cfg.use_graph && !timed) MRAD_TRY(buildGraph(b, P));
-	b->last.trace_nanos = b->last.shade_nanos = b->last.generate_nanos = 0;
-	while (timed && b->stepEvents.size() < 4) {
-		hipEvent_t e;
-		MRAD_HIP(hipEventCreate(&e));
-		b->stepEvents.push_back(e);
-	}
-	// counters: zero, chunks 0 and 1 released (a kernel, not a memset or copy)
-	const uint32_t chunkCount = (uint32_t) b->chunks.size();
-	hipLaunchKernelGGL(wfInitCounters, dim3(1), dim3(64), 0, b->stream, P);
-	MRAD_HIP(hipGetLastError());
-	if (b->cfg.collect_metrics) MRAD_HIP(hipMemsetAsync(b->stats.p, 0, MRAD_STAT_COUNT * 8, b->stream));
-	MRAD_HIP(hipEventRecord(b->evStart, b->stream));
-	uint32_t resolved = 0;        // chunks resolved so far, in order
-	int64_t steps = 0;
-	const int64_t stepGuard = 64 + 4 * 33 * ((int64_t) (b->totalPaths / 64) + chunkCount + 1);
-	for (;;) {
-		if (timed) {
-			for (int cur = 0; cur < 2; cur++) {
-				MRAD_TRY(enqueueStep(b, P, cur, b->stepEvents.data()));
-				MRAD_TRY(accumulateStep(b, b->stepEvents.data()));
-			}
-		} else if (b->graph) {
-			MRAD_HIP(hipGraphLaunch(b->graph, b->stream));
-		} else {
-			MRAD_TRY(enqueueStep(b, P, 0));
-			MRAD_TRY(enqueueStep(b, P, 1));
-		}
-		steps += 2;
-		MRAD_HIP(hipMemcpyAsync(b->hostCounters, b->counters.p, kCounterWords * 4, hipMemcpyDeviceToHost, b->stream));
-		MRAD_HIP(hipStreamSynchronize(b->stream));
-		// Every chunk whose paths have all finished is resolved, in order; its
-		// buffer's count is reset and the chunk two ahead may then be generated
-		// into it.  Both kernels are stream-ordered before the next graph launch.
-		// Only chunks released before this snapshot can be complete: the count
-		// word of chunk r + 2 still holds chunk r's count until the release.
-		const uint32_t *c = b->hostCounters;
-		const uint32_t before = resolved;
-		while (resolved < chunkCount && resolved < c[kGenLimit] &&
-		       c[kDone0 + (resolved & 1u)] == b->chunks[resolved].pathCount) {
-			const uint32_t px = b->chunks[resolved].pixelCount;
-			const int blocks = std::max(1, std::min(16384, (int) ((px + kBlock - 1) / kBlock)));
-			hipLaunchKernelGGL(wfResolve, dim3(blocks), dim3(kBlock), 0, b->stream, P, resolved);
-			MRAD_HIP(hipGetLastError());
-			resolved++;
-		}
-		if (resolved == chunkCount) break;
-		if (resolved != before) {
-			hipLaunchKernelGGL(wfRelease, dim3(1), dim3(64), 0, b->stream, P, before, resolved);
-			MRAD_HIP(hipGetLastError());
-		}
-		if (steps > stepGuard) {
-			const uint32_t r = resolved;
-			return set_error(MRAD_ERR_ILLEGAL_STATE,
-			                 "wavefront did not drain: " + std::to_string(r) + " of " + std::to_string(chunkCount) +
-			                     " chunks resolved (chunk " + std::to_string(r) + ": " +
-			                     std::to_string(b->chunks[r].pathCount) + " paths, " +
-			                     std::to_string(c[kDone0 + (r & 1u)]) + " finished); pools " + std::to_string(c[kCount0]) +
-			                     " / " + std::to_string(c[kCount1]) + ", generating chunk " + std::to_string(c[kGenChunk]) +
-			                     " at " + std::to_string(c[kGenOffset]) + " (limit " + std::to_string(c[kGenLimit]) +
-			                     "), finished " + std::to_string(c[kDone0]) + " / " + std::to_string(c[kDone1]));
-		}
-	}
-	MRAD_HIP(hipEventRecord(b->evEnd, b->stream));
-	MRAD_HIP(hipEventSynchronize(b->evEnd));
-	float ms = 0.0f;
-	MRAD_HIP(hipEventElapsedTime(&ms, b->evStart, b->evEnd));
-	b->last.kernel_nanos = (int64_t) (ms * 1.0e6f);
-	b->last.steps = steps;
-	b->last.paths = b->totalPaths;
-	if (b->cfg.collect_metrics) MRAD_HIP(hipMemcpy(b->last.counters, b->stats.p, sizeof b->last.counters, hipMemcpyDeviceToHost));
-	else std::memset(b->last.counters, 0, sizeof b->last.counters);
-	b->last.total_nanos = nowNanos() - t0;
-	return MRAD_OK;
+	return renderLocked(b);
 }
+
 
 // Changes the launch parameters of a batch (queue capacity, waves per CU, LDS
 // split, graph use, timing, metrics) keeping its scenes resident; device,
@@ -1337,3 +1349,73 @@ int mrad_batch_get_stats(mrad_batch *b, mrad_batch_stats *out) {
 }
 
 }  // extern "C"
+
+namespace mrad {
+
+// Pool size of the wavefront form of a frame: the frame's paths rounded up to
+// whole waves, at most 2^23 (1.3 GB of pools and hit records).
+static uint32_t framePool(uint64_t paths) {
+	const uint64_t cap = std::min<uint64_t>(1ull << 23, std::max<uint64_t>(64, (paths + 63) & ~63ull));
+	return (uint32_t) cap;
+}
+
+int wavefrontFrame(mrad_batch **batch, const mrad_tracer_config &cfg, const DevScene &S, const SceneRange &R,
+                   const WavefrontView &v, float *out, mrad_batch_stats *stats) {
+	const uint64_t pixels = (uint64_t) v.width * (uint64_t) v.height;
+	const uint64_t paths = pixels * (uint64_t) v.spp;
+	if (!*batch) {
+		mrad_wavefront_config c{};
+		c.device = cfg.device;
+		c.queue_capacity = framePool(paths);
+		c.bvh_stack_size = cfg.bvh_stack_size;
+		c.collect_metrics = cfg.collect_metrics;
+		c.use_graph = 1;
+		MRAD_TRY(mrad_batch_create(&c, batch));
+	}
+	mrad_batch *b = *batch;
+	MRAD_TRY(requireBatch(b));
+	if (b->cfg.collect_metrics != cfg.collect_metrics) {
+		mrad_wavefront_config c = b->cfg;
+		c.collect_metrics = cfg.collect_metrics;
+		MRAD_TRY(mrad_batch_configure(b, &c));
+	}
+	std::lock_guard<std::mutex> g(b->mu);
+	MRAD_HIP(hipSetDevice(b->cfg.device));
+	WfScene s{};
+	s.R = R;
+	std::memcpy(s.cam, v.cam, sizeof s.cam);
+	s.width = v.width;
+	s.height = v.height;
+	s.spp = v.spp;
+	s.maxDepth = v.maxDepth;
+	s.rrStart = v.rrStart;
+	s.frameIndex = v.frameIndex;
+	s.frameSeed = v.frameSeed;
+	const bool sameShape = b->scenes.size() == 1 && b->scenes[0].width == v.width &&
+	                       b->scenes[0].height == v.height && b->scenes[0].spp == v.spp;
+	const bool sameScene = b->external && std::memcmp(&b->externalView, &S, sizeof S) == 0 &&
+	                       std::memcmp(&b->scenes[0].R, &R, sizeof R) == 0;
+	b->external = true;
+	b->externalView = S;
+	if (!sameShape) {
+		b->scenes.assign(1, s);
+		b->totalPaths = paths;
+		b->totalPixels = pixels;
+		b->maxSpp = v.spp;
+		b->dirty = true;
+	} else {
+		b->scenes[0] = s;
+		// the graph reads the scene table from memory: a new camera, seed or
+		// frame index needs no new graph, a new arena does (its pointers are
+		// the graph's kernel arguments)
+		if (!sameScene) dropGraph(b);
+		if (!b->dirty) MRAD_TRY(upload(b->stream, b->sceneTable, b->scenes));
+	}
+	MRAD_TRY(renderLocked(b));
+	MRAD_HIP(hipMemcpyAsync(out, b->images.p, (size_t) pixels * 12, hipMemcpyDeviceToDevice, b->stream));
+	MRAD_HIP(hipStreamSynchronize(b->stream));
+	*stats = b->last;
+	return MRAD_OK;
+}
+
+}  // namespace mrad

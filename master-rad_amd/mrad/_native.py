@@ -26,6 +26,7 @@ MRAD_BVH_UNIFORM_SAH = 0
 MRAD_BVH_WEIGHTED_SAH = 1
 MRAD_PRIM_SPHERE, MRAD_PRIM_BOX, MRAD_PRIM_AFFINE_SPHERE, MRAD_PRIM_AFFINE_BOX = 0, 1, 2, 3
 STAT_COUNT = 14
+MRAD_RENDER_FORM_MEGAKERNEL, MRAD_RENDER_FORM_WAVEFRONT = 0, 1
 STAT_NAMES = ("rays", "aabb_tests", "primitive_tests", "sphere_tests", "box_tests", "plane_tests",
               "affine_sphere_tests", "affine_box_tests", "stack_overflows", "max_stack_size",
               "internal_node_visits", "leaf_node_visits", "homogeneous_leaf_node_visits",
@@ -93,6 +94,7 @@ SIGNATURES = {
     "mrad_upload_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneArrays), C.c_uint64]),
     "mrad_set_russian_roulette": (C.c_int, [C.c_void_p, C.c_int]),
     "mrad_set_leaf_batch": (C.c_int, [C.c_void_p, C.c_int]),
+    "mrad_set_render_form": (C.c_int, [C.c_void_p, C.c_int]),
     "mrad_render_sample": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64]),
     "mrad_render_launch_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mrad_copy_frame": (C.c_int, [C.c_void_p, fp]),

@@ -123,6 +123,7 @@ class GpuRayTracer:
         self.height = max(1, int(height))
         self.samples_per_frame = max(1, int(samples_per_frame))
         self.collect_metrics = bool(collect_metrics)
+        self.render_form = N.MRAD_RENDER_FORM_MEGAKERNEL
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -158,6 +159,15 @@ class GpuRayTracer:
         time; same images and counters, faster on leaf-heavy scenes."""
         N.check(N.lib.mrad_set_leaf_batch(self._handle(), int(lanes)))
 
+    def set_render_form(self, form) -> None:
+        """"megakernel" (0, the default: the reference's per-pixel kernel and launch contract) or "wavefront" (1):
+        the same frame, bit for bit with the same counters, rendered by the wavefront form's bounded path pool over
+        the resident scene; its physical launches are bounce steps.  No counterpart in the reference."""
+        forms = {"megakernel": N.MRAD_RENDER_FORM_MEGAKERNEL, "wavefront": N.MRAD_RENDER_FORM_WAVEFRONT}
+        value = forms.get(form, -1) if isinstance(form, str) else int(form)
+        N.check(N.lib.mrad_set_render_form(self._handle(), value))
+        self.render_form = value
+
     def ensure_scene_resident(self, scene: GpuScene) -> None:
         N.check(N.lib.mrad_upload_scene(self._handle(), C.byref(scene.arrays()), C.c_uint64(scene.key)))
 
@@ -171,7 +181,8 @@ class GpuRayTracer:
                                         f"RGB output must contain at least {self.width * self.height * 3} floats")
         if scene is None or camera is None:
             raise N.InvalidArgument(N.MRAD_ERR_INVALID_ARGUMENT, "GPU scene and camera must not be null")
-        self.render_launch_count()
+        if self.render_form == N.MRAD_RENDER_FORM_MEGAKERNEL:
+            self.render_launch_count()
         self.ensure_scene_resident(scene)
         cam = camera.packed() if isinstance(camera, GpuCamera) else np.ascontiguousarray(camera, np.float32)
         N.check(N.lib.mrad_render_sample(self._handle(), N.ptr(rgb, N.fp), N.ptr(cam, N.fp), int(max_depth),

@@ -193,3 +193,22 @@ def test_render_launch_plan_limits_are_checked_before_the_device():
                dict(replay_rays_per_launch=262_145), dict(replay_rays_per_launch=-1)):
         with pytest.raises(mrad.InvalidArgument):
             mrad.GpuRayTracer(8, 8, 1, True, **kw)
+
+
+def test_render_form_checks_arguments_first():
+    """mrad_set_render_form: 0 or 1, checked before the device; the wavefront form skips the megakernel's pixel-tile
+    plan (a plan of 16384 launches is then no error) and still needs a device to render."""
+    t = mrad.GpuRayTracer(128, 128, 1, True, pixels_per_launch=1)
+    for bad in (2, -1, "fast"):
+        with pytest.raises(mrad.InvalidArgument, match="render form"):
+            t.set_render_form(bad)
+    t.set_render_form("wavefront")
+    sc = mrad.GpuScene()
+    sc.add_spheres([[0, 0, 0, 1]], mrad.material_array((0.5, 0.5, 0.5)))
+    cam = mrad.GpuCamera.look_at((0, 0, -3), (0, 0, 0), 40)
+    with pytest.raises(mrad.IllegalState):
+        t.render_sample(np.zeros(128 * 128 * 3, np.float32), sc, cam, 4, 0, 1)
+    t.set_render_form("megakernel")
+    with pytest.raises(mrad.InvalidArgument, match="exceeding the limit of 8192"):
+        t.render_sample(np.zeros(128 * 128 * 3, np.float32), sc, cam, 4, 0, 1)
+    t.close()
