@@ -27,7 +27,14 @@
 
 namespace mrad {
 
-constexpr int kAdjTile = 16;
+// Block of the backward kernel: MRAD_ADJOINT_BLOCK threads in 8x8-pixel
+// waves, 256 (a 16x16 tile of 4 waves) or 64 (one 8x8 wave per block).
+#ifndef MRAD_ADJOINT_BLOCK
+#define MRAD_ADJOINT_BLOCK 64
+#endif
+constexpr int kAdjBlock = MRAD_ADJOINT_BLOCK;
+static_assert(kAdjBlock == 64 || kAdjBlock == 256, "adjoint block is one or four waves");
+constexpr int kAdjTile = kAdjBlock == 256 ? 16 : 8;
 constexpr int kMaxVertices = 32;
 
 struct AdjointParams {
@@ -42,12 +49,12 @@ struct AdjointParams {
 };
 
 template <int kCap>
-__global__ __launch_bounds__(kBlock) void adjointKernel(AdjointParams P) {
+__global__ __launch_bounds__(kAdjBlock) void adjointKernel(AdjointParams P) {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int tileX = blockIdx.x % P.tilesX, tileY = blockIdx.x / P.tilesX;
 	const int x = tileX * kAdjTile + (wave & 1) * 8 + (lane & 7);
 	const int y = tileY * kAdjTile + (wave >> 1) * 8 + (lane >> 3);
-	__shared__ StackLds stackLds;
+	__shared__ StackLdsT<kShortStack, kAdjBlock / 64> stackLds;
 	const LdsStack lds = laneStack(stackLds);
 	if (x >= P.width || y >= P.height) return;
 	const uint32_t pixel = (uint32_t) y * (uint32_t) P.width + (uint32_t) x;
@@ -214,8 +221,8 @@ int mrad_backward(mrad_tracer *t, const float *gradImage, const float camera[12]
 	const int blocks = P.tilesX * ((H + kAdjTile - 1) / kAdjTile);
 	const int64_t t0 = nowNanos();
 	MRAD_HIP(hipEventRecord(t->evStart, t->stream));
-	if (t->cfg.bvh_stack_size <= 64) hipLaunchKernelGGL(adjointKernel<64>, dim3(blocks), dim3(kBlock), 0, t->stream, P);
-	else hipLaunchKernelGGL(adjointKernel<256>, dim3(blocks), dim3(kBlock), 0, t->stream, P);
+	if (t->cfg.bvh_stack_size <= 64) hipLaunchKernelGGL(adjointKernel<64>, dim3(blocks), dim3(kAdjBlock), 0, t->stream, P);
+	else hipLaunchKernelGGL(adjointKernel<256>, dim3(blocks), dim3(kAdjBlock), 0, t->stream, P);
 	MRAD_HIP(hipGetLastError());
 	MRAD_HIP(hipEventRecord(t->evEnd, t->stream));
 	MRAD_HIP(hipEventSynchronize(t->evEnd));

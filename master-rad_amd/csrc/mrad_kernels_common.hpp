@@ -12,19 +12,19 @@ constexpr int kBlock = 256;          // 4 wave64s
 constexpr int kWaves = kBlock / 64;
 
 // Per-block LDS for the short traversal stacks (kShort entries per lane) of
-// its 4 waves.
-template <int kShort = kShortStack>
+// its kW waves.
+template <int kShort = kShortStack, int kW = kWaves>
 struct StackLdsT {
 #if MRAD_STACK_PACKED
-	unsigned long long slot[kWaves * (kShort > 0 ? kShort : 1) * 64];
+	unsigned long long slot[kW * (kShort > 0 ? kShort : 1) * 64];
 #else
-	int node[kWaves * (kShort > 0 ? kShort : 1) * 64];
-	float entry[kWaves * (kShort > 0 ? kShort : 1) * 64];
+	int node[kW * (kShort > 0 ? kShort : 1) * 64];
+	float entry[kW * (kShort > 0 ? kShort : 1) * 64];
 #endif
 };
 using StackLds = StackLdsT<>;
-template <int kShort>
-__device__ __forceinline__ LdsStack laneStack(StackLdsT<kShort> &s) {
+template <int kShort, int kW>
+__device__ __forceinline__ LdsStack laneStack(StackLdsT<kShort, kW> &s) {
 	const int base = (threadIdx.x >> 6) * kShort * 64 + (threadIdx.x & 63);
 #if MRAD_STACK_PACKED
 	return {(LdsSlot *) (s.slot + base)};
