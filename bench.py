@@ -233,17 +233,20 @@ def bench_batch(args) -> dict:
     total_paths = sum(w.paths for w in ws)
     prims = sum(w.scene.primitive_count() for w in ws)
     log(f"built 64 scenes ({prims} primitives, {total_paths} paths) in {time.perf_counter() - t0:.1f} s")
-    cfg = dict(queue_capacity=args.queue_capacity, waves_per_cu=args.waves_per_cu)
+    cfg = dict(queue_capacity=args.queue_capacity, waves_per_cu=args.waves_per_cu, trace_passes=args.trace_passes,
+               pass_pops=args.pass_pops)
     b = make_batch(ws, **cfg)
     tuned = None
     if args.autotune_budget > 0:
         from mrad.autotune import tune
         t1 = time.perf_counter()
         grid = {"queue_capacity": [1 << 24, 1 << 25, 1 << 26], "waves_per_cu": [24, 28, 32],
-                "lds_stack_entries": [4, 10], "stack_storage": [32, 64]}
+                "lds_stack_entries": [4, 10], "stack_storage": [32, 64], "trace_passes": [1, 2],
+                "pass_pops": [96, 128, 192]}
         best, table = tune(b, grid=grid, log=log, budget_s=args.autotune_budget)
         cfg = dict(queue_capacity=best.queue_capacity, waves_per_cu=best.waves_per_cu,
-                   lds_stack_entries=best.lds_stack_entries, stack_storage=best.stack_storage)
+                   lds_stack_entries=best.lds_stack_entries, stack_storage=best.stack_storage,
+                   trace_passes=best.trace_passes, pass_pops=best.pass_pops)
         tuned = {"best": cfg, "grid": {k: v for k, v in grid.items()}, "trials": table,
                  "budget_seconds": args.autotune_budget, "seconds": time.perf_counter() - t1}
         log(f"autotuned in {tuned['seconds']:.0f} s: {cfg}")
@@ -286,15 +289,21 @@ def bench_batch(args) -> dict:
     mk_value = total_paths / mk_wall / 1e6
     log(f"megakernel: {mk_wall * 1e3:.0f} ms for 64 renders, images bit-identical: {identical}")
 
-    # dominant kernel: per-kernel event timing and exact counters of the same batch
-    bt = make_batch(ws, time_kernels=True, **cfg)
+    # dominant kernel: per-kernel event timing and exact counters of the same batch, in the configuration the
+    # committed counter profile was taken with (its per-dispatch counters and these times describe the same
+    # launches; the trace runs trace_passes launches per bounce step)
+    from mrad.profile_run import BATCH_PROFILE_CONFIG
+    prof_cfg = dict(BATCH_PROFILE_CONFIG)
+    passes = prof_cfg.get("trace_passes", 1)
+    bt = make_batch(ws, time_kernels=True, **prof_cfg)
     st_t = bt.render()
     bt.close()
     bm = make_batch(ws, collect_metrics=True, **cfg)
     st_m = bm.render()
     bm.close()
     trace_s = st_t["trace_nanos"] / 1e9
-    launches = max(1, st_t["steps"])
+    launches = max(1, st_t["steps"] * passes)
+    trace_kernel = "wfTracePass" if passes > 1 else "wfTrace"
     log(f"trace kernel {trace_s * 1e3:.0f} ms of {st_t['kernel_nanos'] / 1e6:.0f} ms over {launches} launches")
 
     # the same batch without roulette: every path runs its 6 bounces
@@ -346,9 +355,9 @@ def bench_batch(args) -> dict:
         "adjoint": adj,
         "adjoint_over_forward": adj["ratio"],
         "gi_test_context": gi,
-        "roofline": roofline_entry("wfTrace", trace_s / launches, launches, trace_s * 1e3,
-                                   algorithmic_bytes(st_m) / launches, load_traffic("batch-64"),
-                                   load_counters("batch-64"), "batch-64"),
+        "roofline": {**roofline_entry(trace_kernel, trace_s / launches, launches, trace_s * 1e3,
+                                      algorithmic_bytes(st_m) / launches, load_traffic("batch-64"),
+                                      load_counters("batch-64"), "batch-64"), "render_config": prof_cfg},
         "counters_per_step": {k: st_m[k] for k in mrad.STAT_NAMES},
     }, ws
 
@@ -659,6 +668,9 @@ def main():
     ap.add_argument("--workload", choices=["batch", "het", "clustered", "hero"], default="batch")
     ap.add_argument("--queue-capacity", type=int, default=1 << 26)
     ap.add_argument("--waves-per-cu", type=int, default=28)
+    ap.add_argument("--trace-passes", type=int, default=2,
+                    help="closest-hit passes per bounce step of the batch (capped passes regroup long traversals)")
+    ap.add_argument("--pass-pops", type=int, default=128, help="pop budget of a capped trace pass")
     ap.add_argument("--random-layouts", type=int, default=10,
                     help="layouts per random F1-F4 row timed by --workload hero (the study used 100)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

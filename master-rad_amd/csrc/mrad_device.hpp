@@ -567,6 +567,9 @@ __device__ __forceinline__ float entryDistanceFastNz(float lx, float ly, float l
 // f is non-decreasing (RN of a product with a positive constant is
 // monotone, and f(t) > 0 >= f(t') for t > 0 >= t'), so
 // min(cb, f(b.x), f(b.y), f(b.z)) = min(cb, f(min(b.x, b.y, b.z))).
+#ifndef MRAD_ENTRY_LEAN
+#define MRAD_ENTRY_LEAN 1
+#endif
 // The near (a) and far (b) slab distances of one axis.
 __device__ __forceinline__ void slabAxisBf(float oc, float dc, float inv, float lo, float hi, bool nz, float &a,
                                            float &b) {
@@ -595,8 +598,21 @@ __device__ __forceinline__ float entryDistanceFastBf(float lx, float ly, float l
 	// kernel's scalar budget pays for with spill reloads inside the loop
 	const float tMin = fmaxf(fmaxf(fmaxf(kEps, ax), ay), az);
 	const float bMin = fminf(fminf(bx, by), bz);
+#if MRAD_ENTRY_LEAN
+	// Two checks of the reference's form are dropped because they cannot
+	// change the result here (no value below is NaN: the bounds, origin and
+	// reciprocals are finite, and a zero direction component yields +-inf
+	// through selects only):
+	// * the widening's sign test: for bMin <= 0 the unwidened and the widened
+	//   far distance are both <= 0 < kEps <= tMin, so the box is culled
+	//   either way; for bMin > 0 the product is the reference's;
+	// * the final tMin <= cb: tMax = min(cb, ..) >= tMin already implies it.
+	const float tMax = fminf(cb, bMin * 1.00000072f);
+	return tMax < tMin ? INFINITY : tMin;
+#else
 	const float tMax = fminf(cb, bMin > 0.0f ? bMin * 1.00000072f : bMin);
 	return (tMax < tMin || !(tMin <= cb)) ? INFINITY : tMin;
+#endif
 }
 #ifndef MRAD_ENTRY_BRANCH_FREE
 #define MRAD_ENTRY_BRANCH_FREE 1
@@ -892,6 +908,101 @@ __device__ __forceinline__ void orderChildren(int4 nd, float le, float re, int &
 	farE = swap ? le : re;
 }
 
+// The default traversal loop of traceBvh (GpuKernelSources.java:895-958):
+// pops and expands stack entries until the stack is empty or overflows, or,
+// with kBudget, until `budget` pops have been made.  The query's state
+// (stack, size, overflow, best) is then left in `b` and `best` as the next
+// pop needs it, so a later call continues the same query with the same
+// pops, tests and counters.
+template <bool kMetrics, int kCap, int kShort, int kSlab, int kTop, bool kBudget>
+__device__ __forceinline__ void bvhLoop(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
+                                        Counters<kMetrics> &c, BvhState<kCap, kShort> &b, TopNodes top,
+                                        int budget) {
+	const int cap = R.stackLimit;
+	int size = b.size;
+	bool overflow = b.overflow;
+	float ch = cullBound(best.t);   // changes only when a leaf improves best.t
+	while (!overflow && size > 0) {
+		if constexpr (kBudget) {
+			if (budget <= 0) break;
+			budget--;
+		}
+		--size;
+		int node;
+		float entry;
+		b.st.get(size, node, entry);
+		if (entry > ch) continue;
+		const float4 *rec = S.nodes + 4 * (size_t) (uint32_t) node;
+#if MRAD_NODE_PREFETCH
+		const float4 a = rec[0], bb = rec[1], cc = rec[2];
+		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
+#else
+		float4 a, bb, cc;
+		int4 nd;
+		const uint32_t rel = (uint32_t) node - (uint32_t) top.base;
+		const bool inTop = kTop > 0 && rel < (uint32_t) top.count;
+		if (inTop) {
+			const LdsFloat4 *q = top.p + 4 * rel;
+			const float4 w = ldsLoad4(q + 3);
+			nd = make_int4(__float_as_int(w.x), __float_as_int(w.y), __float_as_int(w.z), __float_as_int(w.w));
+			a = ldsLoad4(q);
+			bb = ldsLoad4(q + 1);
+			cc = ldsLoad4(q + 2);
+		} else {
+			nd = *reinterpret_cast<const int4 *>(rec + 3);
+		}
+#endif
+		if (nd.x < 0) {
+			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
+			ch = cullBound(best.t);
+			continue;
+		}
+		c.add(10, 1);
+#if !MRAD_NODE_PREFETCH
+		if (!inTop) {
+			a = rec[0];
+			bb = rec[1];
+			cc = rec[2];
+		}
+#endif
+		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
+		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
+		c.add(1, 2);
+		int nearN, farN;
+		float nearE, farE;
+		orderChildren(nd, le, re, nearN, farN, nearE, farE);
+#if MRAD_PUSH_BRANCH_FREE
+		// Both children are written unconditionally, the nearer one above the
+		// farther if that is finite; a non-finite child's slot lies past the
+		// new top (storage holds cap + 1 entries) and is never read.  size <=
+		// cap - 1 after the pop, so only two finite children overflow, after
+		// the far push brought the maximum to cap.
+		const int s1 = size + (isfinite(farE) ? 1 : 0);
+		const int s2 = s1 + (isfinite(nearE) ? 1 : 0);
+		if (s2 > cap) { c.maxStack(cap); overflow = true; break; }
+		b.st.set(size, farN, farE);
+		b.st.set(s1, nearN, nearE);
+		size = s2;
+		c.maxStack(size);
+#else
+		if (isfinite(farE)) {
+			if (size >= cap) { overflow = true; break; }
+			b.st.set(size, farN, farE);
+			size++;
+			c.maxStack(size);
+		}
+		if (isfinite(nearE)) {
+			if (size >= cap) { overflow = true; break; }
+			b.st.set(size, nearN, nearE);
+			size++;
+			c.maxStack(size);
+		}
+#endif
+	}
+	b.size = size;
+	b.overflow = overflow;
+}
+
 template <bool kMetrics, int kCap, int kShort = kShortStack, int kSlab = kSlabAny, int kBatch = 0, int kTop = 0>
 __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
                                 Counters<kMetrics> &c, LdsStack lds, TopNodes top = {}) {
@@ -1024,79 +1135,11 @@ __device__ inline void traceBvh(const DevScene &S, const SceneRange &R, const Ra
 #endif
 	}
 #else
-	while (!overflow && size > 0) {
-		--size;
-		int node;
-		float entry;
-		b.st.get(size, node, entry);
-		if (entry > ch) continue;
-		const float4 *rec = S.nodes + 4 * (size_t) (uint32_t) node;
-#if MRAD_NODE_PREFETCH
-		const float4 a = rec[0], bb = rec[1], cc = rec[2];
-		const int4 nd = *reinterpret_cast<const int4 *>(rec + 3);
-#else
-		float4 a, bb, cc;
-		int4 nd;
-		const uint32_t rel = (uint32_t) node - (uint32_t) top.base;
-		const bool inTop = kTop > 0 && rel < (uint32_t) top.count;
-		if (inTop) {
-			const LdsFloat4 *q = top.p + 4 * rel;
-			const float4 w = ldsLoad4(q + 3);
-			nd = make_int4(__float_as_int(w.x), __float_as_int(w.y), __float_as_int(w.z), __float_as_int(w.w));
-			a = ldsLoad4(q);
-			bb = ldsLoad4(q + 1);
-			cc = ldsLoad4(q + 2);
-		} else {
-			nd = *reinterpret_cast<const int4 *>(rec + 3);
-		}
-#endif
-		if (nd.x < 0) {
-			testLeaf<kMetrics, kSlab>(S, R, rec, nd, ray, best, c);
-			ch = cullBound(best.t);
-			continue;
-		}
-		c.add(10, 1);
-#if !MRAD_NODE_PREFETCH
-		if (!inTop) {
-			a = rec[0];
-			bb = rec[1];
-			cc = rec[2];
-		}
-#endif
-		const float le = entryDistanceCb<kSlab>(a.x, a.y, a.z, a.w, bb.x, bb.y, ray, ch);
-		const float re = entryDistanceCb<kSlab>(bb.z, bb.w, cc.x, cc.y, cc.z, cc.w, ray, ch);
-		c.add(1, 2);
-		int nearN, farN;
-		float nearE, farE;
-		orderChildren(nd, le, re, nearN, farN, nearE, farE);
-#if MRAD_PUSH_BRANCH_FREE
-		// Both children are written unconditionally, the nearer one above the
-		// farther if that is finite; a non-finite child's slot lies past the
-		// new top (storage holds cap + 1 entries) and is never read.  size <=
-		// cap - 1 after the pop, so only two finite children overflow, after
-		// the far push brought the maximum to cap.
-		const int s1 = size + (isfinite(farE) ? 1 : 0);
-		const int s2 = s1 + (isfinite(nearE) ? 1 : 0);
-		if (s2 > cap) { c.maxStack(cap); overflow = true; break; }
-		b.st.set(size, farN, farE);
-		b.st.set(s1, nearN, nearE);
-		size = s2;
-		c.maxStack(size);
-#else
-		if (isfinite(farE)) {
-			if (size >= cap) { overflow = true; break; }
-			b.st.set(size, farN, farE);
-			size++;
-			c.maxStack(size);
-		}
-		if (isfinite(nearE)) {
-			if (size >= cap) { overflow = true; break; }
-			b.st.set(size, nearN, nearE);
-			size++;
-			c.maxStack(size);
-		}
-#endif
-	}
+	b.size = size;
+	b.overflow = overflow;
+	bvhLoop<kMetrics, kCap, kShort, kSlab, kTop, false>(S, R, ray, best, c, b, top, 0);
+	size = b.size;
+	overflow = b.overflow;
 #endif
 	}
 	b.size = size;

@@ -73,7 +73,16 @@ struct WfPool {
 // finished paths of the chunk in each result buffer.
 enum {
 	kCount0 = 0, kCount1 = 1, kGenChunk = 2, kGenOffset = 3, kGenLimit = 4, kWorkTrace = 5, kDone0 = 6, kDone1 = 7,
+	kSuspend0 = 8, kSuspend1 = 9, kWorkPass1 = 10, kWorkPass2 = 11,
 	kCounterWords = 16
+};
+
+// Continuation list of the capped trace passes: per suspended closest-hit
+// query its pool slot, best distance bits, best rank and stack size (hdr),
+// and its stack entries, entry j of item k at stack[j * capacity + k].
+struct WfCont {
+	uint4 *hdr;
+	unsigned long long *stack;
 };
 constexpr uint32_t kDead = 0xFFFFu;          // depth field of a path that never traces
 constexpr uint32_t kMissKind = 7u;
@@ -93,6 +102,8 @@ struct WfParams {
 	uint32_t *counters;
 	uint32_t capacity;
 	unsigned long long *stats;
+	WfCont cont[2];         // continuation lists of the capped trace passes
+	uint32_t contCap;       // items per list
 };
 
 __device__ __forceinline__ int sceneOfPath(const WfParams &P, uint64_t g) {
@@ -227,6 +238,10 @@ __global__ void wfAdvanceGenerate(WfParams P, int cur) {
 		P.counters[kGenChunk] = done ? chunk + 1u : chunk;
 	}
 	P.counters[kWorkTrace] = 0;
+	P.counters[kWorkPass1] = 0;
+	P.counters[kWorkPass2] = 0;
+	P.counters[kSuspend0] = 0;
+	P.counters[kSuspend1] = 0;
 }
 
 // Counter block at the start of a render: zero, with chunks 0 and 1 released.
@@ -269,6 +284,58 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 	return __shfl(base, 0, 64);
 }
 
+// Work distribution of the trace kernel.  The waves of one block share a CU,
+// and so its L1: a block takes the pool in super-chunks of kTraceSuper
+// consecutive paths (neighbouring pixels, or paths that were neighbours a
+// bounce earlier), its waves taking 64-path chunks of the current one in
+// turn from a block counter in LDS, so that their node fetches hit the same
+// lines.  The first kTraceStaticPct % of the pool (whole rounds of
+// super-chunks) is dealt out statically and interleaved: block b takes
+// super-chunks b, b + B, b + 2B, ... of the B blocks, so every block's share
+// samples the whole pool and the shares cost about the same.  The rest is a
+// global queue of 64-path chunks that evens out what is left.  kTraceSuper 0
+// keeps the single global queue.
+#ifndef MRAD_TRACE_SUPER
+#define MRAD_TRACE_SUPER 2048
+#endif
+#ifndef MRAD_TRACE_STATIC_PCT
+#define MRAD_TRACE_STATIC_PCT 70
+#endif
+constexpr uint32_t kTraceSuper = MRAD_TRACE_SUPER;
+static_assert(kTraceSuper % 64 == 0, "super-chunks hold whole 64-path chunks");
+
+struct TraceWork {
+	uint32_t perBlock;    // paths of the static part per block (whole super-chunks)
+	uint32_t staticEnd;   // end of the static part: perBlock * gridDim.x
+};
+__device__ __forceinline__ TraceWork traceWork(uint32_t count) {
+	TraceWork w{0u, 0u};
+	if constexpr (kTraceSuper > 0) {
+		const uint64_t round = (uint64_t) gridDim.x * kTraceSuper;
+		const uint64_t rounds = (uint64_t) count * MRAD_TRACE_STATIC_PCT / 100u / round;
+		w.perBlock = (uint32_t) (rounds * kTraceSuper);
+		w.staticEnd = (uint32_t) (rounds * round);
+	}
+	return w;
+}
+// Next 64-path chunk of this wave: from the block's static share while it
+// lasts (`next` is the block's LDS counter), then from the global queue.
+__device__ __forceinline__ uint32_t fetchTraceChunk(uint32_t *next, uint32_t *queue, const TraceWork &w) {
+	if constexpr (kTraceSuper == 0) {
+		return fetchChunk(queue);
+	} else {
+		uint32_t base = 0;
+		if ((threadIdx.x & 63) == 0) {
+			const uint32_t off = atomicAdd(next, 64u);
+			if (off < w.perBlock)
+				base = ((off / kTraceSuper) * gridDim.x + blockIdx.x) * kTraceSuper + off % kTraceSuper;
+			else
+				base = w.staticEnd + atomicAdd(queue, 64u);
+		}
+		return __shfl(base, 0, 64);
+	}
+}
+
 // Occupancy the trace kernels are compiled for (waves per SIMD; the register
 // budget is 512 / this).  7 waves (72 VGPRs) with the packed stack slots and
 // the branch-free push is 1.8 % faster than 8 waves (64 VGPRs), where both
@@ -305,18 +372,24 @@ void wfTrace(WfParams P, int cur) {
 	constexpr int kTop = kTraceTop;
 	__shared__ float4 topLds[kWaves][kTop > 0 ? 4 * kTop : 1];
 	__shared__ uint32_t doneLds[2];
+	__shared__ uint32_t nextLds;
 	const uint32_t count = P.counters[kCount0 + cur];
 	const WfPool Q = P.pool[cur];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const unsigned long long below = (1ull << lane) - 1ull;
 	uint32_t finished[2] = {0u, 0u};
 	if (kFuse && threadIdx.x < 2) doneLds[threadIdx.x] = 0u;
+	const TraceWork work = traceWork(count);
+	if constexpr (kTraceSuper > 0) {
+		if (threadIdx.x == 0) nextLds = 0u;
+		__syncthreads();
+	}
 	int cached = -1;
 	TopNodes top{(const LdsFloat4 *) topLds[wave], 0, 0};
 	Counters<kMetrics> c;
 	c.zero();
 	for (;;) {
-		const uint32_t base = fetchChunk(P.counters + kWorkTrace);
+		const uint32_t base = fetchTraceChunk(&nextLds, P.counters + kWorkTrace, work);
 		if (base >= count) break;
 		const uint32_t i = base + lane;
 		uint4 meta = make_uint4(0, 0, 0xFFFFFFFFu, 0);
@@ -427,6 +500,151 @@ void wfTrace(WfParams P, int cur) {
 		__syncthreads();
 		if (threadIdx.x < 2 && doneLds[threadIdx.x]) atomicAdd(P.counters + kDone0 + threadIdx.x, doneLds[threadIdx.x]);
 	}
+}
+
+// ------------------------------------------------------ capped trace passes
+//
+// The trace of a bounce step in passes.  A wave of the single pass runs until
+// its longest traversal ends, and traversal lengths are heavy-tailed (the
+// batch's secondary rays: ~55 pops on average, up to ~300), so most of its
+// lanes idle while a few finish.  Here pass 0 starts the closest-hit query of
+// every live path of the pool and suspends any query still traversing after
+// `budget` pops: its pool slot, best distance and rank, stack size and stack
+// entries go to a continuation list.  The next pass resumes the suspended
+// queries, 64 to a wave, from that list; the last pass has no budget.  A
+// resumed query makes the same pops in the same order from the same state, so
+// hits, images and counters are those of the single pass.  A query whose
+// stack holds more than kSuspendEntries entries, or that finds its list full,
+// is not suspended: it keeps traversing in the current pass.
+#ifndef MRAD_SUSPEND_ENTRIES
+#define MRAD_SUSPEND_ENTRIES 12
+#endif
+constexpr int kSuspendEntries = MRAD_SUSPEND_ENTRIES;
+static_assert(kSuspendEntries >= 1 && kSuspendEntries <= 32, "suspended stacks fit the smallest storage");
+
+template <bool kMetrics, int kCap, int kShort>
+__device__ __forceinline__ void bvhLoopAnyForm(const DevScene &S, const SceneRange &R, const Ray &ray, Best &best,
+                                               Counters<kMetrics> &c, BvhState<kCap, kShort> &b, int budget) {
+	// one loop per slab-quotient form, as traceClosest chooses them
+	if (ray.fast && ray.d.x != 0.0f && ray.d.y != 0.0f && ray.d.z != 0.0f)
+		bvhLoop<kMetrics, kCap, kShort, kSlabFastNz, 0, true>(S, R, ray, best, c, b, TopNodes{}, budget);
+	else if (ray.fast)
+		bvhLoop<kMetrics, kCap, kShort, kSlabFast, 0, true>(S, R, ray, best, c, b, TopNodes{}, budget);
+	else
+		bvhLoop<kMetrics, kCap, kShort, kSlabIeee, 0, true>(S, R, ray, best, c, b, TopNodes{}, budget);
+}
+
+// Pass `pass` (0, 1 or 2) of a bounce step on pool `cur`; budget <= 0 for the
+// last pass.  Pass p > 0 reads list (p - 1) & 1; pass p writes list p & 1.
+template <bool kMetrics, int kCap, int kShort>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
+void wfTracePass(WfParams P, int cur, int pass, int budget) {
+	__shared__ StackLdsT<kShort> stackLds;
+	const LdsStack lds = laneStack(stackLds);
+	const int lane = threadIdx.x & 63;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	const WfPool Q = P.pool[cur];
+	const WfCont in = P.cont[(pass + 1) & 1], out = P.cont[pass & 1];
+	// a list's count may exceed its capacity: items past it were not stored
+	// (their queries finished in the pass that tried to suspend them)
+	const uint32_t count = pass == 0 ? P.counters[kCount0 + cur]
+	                                 : min(P.counters[kSuspend0 + ((pass + 1) & 1)], P.contCap);
+	uint32_t *work = P.counters + (pass == 0 ? kWorkTrace : pass == 1 ? kWorkPass1 : kWorkPass2);
+	uint32_t *outCount = P.counters + kSuspend0 + (pass & 1);
+	// pass 0 reads the pool in super-chunks, as wfTrace does; a continuation
+	// list has no order worth keeping
+	__shared__ uint32_t nextLds;
+	const TraceWork plan = pass == 0 ? traceWork(count) : TraceWork{0u, 0u};
+	if constexpr (kTraceSuper > 0) {
+		if (threadIdx.x == 0) nextLds = 0u;
+		__syncthreads();
+	}
+	Counters<kMetrics> c;
+	c.zero();
+	for (;;) {
+		const uint32_t base = fetchTraceChunk(&nextLds, work, plan);
+		if (base >= count) break;
+		const uint32_t k = base + lane;
+		bool live = k < count;
+		uint32_t i = k;                   // pool slot of the path
+		uint4 hdr = make_uint4(0u, 0u, 0u, 0u);
+		if (live && pass > 0) {
+			hdr = in.hdr[k];
+			i = hdr.x;
+		}
+		uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+		if (live) meta = Q.meta[i];
+		if (live && pass == 0 && (meta.z >> 16) == kDead) {
+			P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			live = false;
+		}
+		const SceneRange &R = P.scenes[live ? (int) (meta.z & 0xFFFFu) : 0].R;
+		Best best{1e30f, 0xFFFFFFFFu, -1, -1};
+		BvhState<kCap, kShort> b;
+		b.st.lds = lds;
+		b.size = 0;
+		b.overflow = false;
+		Ray ray{};
+		bool traversing = false;
+		if (live) {
+			const float4 a4 = Q.ori[i], d4 = Q.dir[i];
+			ray = makeRay(mk(a4.x, a4.y, a4.z), mk(d4.x, d4.y, d4.z), R.fastSlabs != 0);
+			if (pass == 0) {
+				c.add(0, 1);
+				if (R.useBvh != 0) bvhBegin<kMetrics, kCap, kShort, kSlabAny>(R, ray, best, c, b);
+			} else {
+				// only BVH queries are suspended; rank 0xFFFFFFFF is "no hit yet"
+				// (arena indices stay below kRefIndexMask)
+				best.t = __uint_as_float(hdr.y);
+				best.rank = hdr.z;
+				best.kind = hdr.z == 0xFFFFFFFFu ? -1 : (int) (hdr.z >> 30);
+				best.index = hdr.z == 0xFFFFFFFFu ? -1 : (int) (hdr.z & kRefIndexMask);
+				b.size = (int) hdr.w;
+				for (int j = 0; j < b.size; j++) {
+					const unsigned long long v = in.stack[(size_t) j * P.contCap + k];
+					b.st.set(j, (int) (uint32_t) v, __uint_as_float((uint32_t) (v >> 32)));
+				}
+			}
+			traversing = R.useBvh != 0;
+		}
+		bool suspended = false;
+		int left = budget > 0 ? budget : 0x7FFFFFFF;
+		for (;;) {
+			if (traversing) bvhLoopAnyForm<kMetrics, kCap, kShort>(P.S, R, ray, best, c, b, left);
+			const bool open = traversing && !b.overflow && b.size > 0;
+			const bool want = open && b.size <= kSuspendEntries;
+			const unsigned long long m = __ballot(want);
+			uint32_t first = 0;
+			if (m) {
+				const int leader = __ffsll((long long) m) - 1;
+				if (lane == leader) first = atomicAdd(outCount, (uint32_t) __popcll(m));
+				first = __shfl(first, leader, 64);
+			}
+			const uint32_t slot = first + (uint32_t) __popcll(m & below);
+			if (want && slot < P.contCap) {
+				out.hdr[slot] = make_uint4(i, __float_as_uint(best.t), best.rank, (uint32_t) b.size);
+				for (int j = 0; j < b.size; j++) {
+					int n;
+					float e;
+					b.st.get(j, n, e);
+					out.stack[(size_t) j * P.contCap + slot] =
+					    (uint32_t) n | (unsigned long long) __float_as_uint(e) << 32;
+				}
+				suspended = true;
+			}
+			if (!open || suspended) break;
+			left = 0x7FFFFFFF;            // not suspended: finish in this pass
+		}
+		if (live && !suspended) {
+			if (R.useBvh != 0) bvhEnd<kMetrics>(P.S, R, ray, best, c, b);
+			else traceLinear<kMetrics>(P.S, R, ray, best, c);
+			c.add(2, (uint32_t) R.planeCount);
+			c.add(5, (uint32_t) R.planeCount);
+			tracePlanes(P.S, R, ray.o, ray.d, best);
+			writeHit(P, i, ray.o, ray.d, best, best.kind >= 0);
+		}
+	}
+	if (P.stats) flushCounters<kMetrics>(c, P.stats);
 }
 
 // Shading of one pool entry; true when the path continues (its new state is
@@ -889,6 +1107,27 @@ TraceFn traceFor(int storage, int ldsEntries, bool regroup) {
 	     : storage <= 64 ? wfTrace<M, 64, kShortStack, F> : wfTrace<M, 256, kShortStack, F>;
 }
 
+// The capped passes (trace_passes > 1) of the plain, unfused trace.
+#ifndef MRAD_TRACE_PASSES
+#define MRAD_TRACE_PASSES 1
+#endif
+#ifndef MRAD_PASS_POPS
+#define MRAD_PASS_POPS 128
+#endif
+constexpr int kDefaultTracePasses = MRAD_TRACE_PASSES;
+constexpr int kDefaultPassPops = MRAD_PASS_POPS;
+using TracePassFn = void (*)(WfParams, int, int, int);
+inline bool cappedPasses(const mrad_wavefront_config &c) {
+	return c.trace_passes > 1 && c.leaf_regroup == 0 && !fusedShade(c);
+}
+template <bool M>
+TracePassFn tracePassFor(int storage, int ldsEntries) {
+	if (ldsEntries <= 4)
+		return storage <= 32 ? wfTracePass<M, 32, 4> : storage <= 64 ? wfTracePass<M, 64, 4> : wfTracePass<M, 256, 4>;
+	return storage <= 32 ? wfTracePass<M, 32, kShortStack>
+	     : storage <= 64 ? wfTracePass<M, 64, kShortStack> : wfTracePass<M, 256, kShortStack>;
+}
+
 }  // namespace mrad
 
 using namespace mrad;
@@ -907,6 +1146,7 @@ struct mrad_batch {
 	bool dirty = true;
 	DevArena arena;
 	DevBuf sceneTable, chunkTable, pool[2][4], hitA, hitB, results, images, counters, stats;
+	DevBuf contHdr[2], contStack[2];     // continuation lists of the capped trace passes
 	hipGraphExec_t graph = nullptr;
 	std::vector<hipEvent_t> stepEvents;
 	uint64_t totalPaths = 0, totalPixels = 0;
@@ -992,6 +1232,22 @@ int prepare(mrad_batch *b) {
 	return MRAD_OK;
 }
 
+// Continuation lists of the capped trace passes, sized for the configuration.
+// A buffer that moves (or is released by a failed allocation) drops the
+// captured graph, which holds the old address.
+int ensureSuspend(mrad_batch *b) {
+	if (!cappedPasses(b->cfg)) return MRAD_OK;
+	const size_t n = (size_t) b->cfg.suspend_capacity;
+	int rc = MRAD_OK;
+	for (int q = 0; q < 2 && rc == MRAD_OK; q++) {
+		const void *h = b->contHdr[q].p, *st = b->contStack[q].p;
+		rc = ensure(b->contHdr[q], n * 16);
+		if (rc == MRAD_OK) rc = ensure(b->contStack[q], n * kSuspendEntries * 8);
+		if (b->contHdr[q].p != h || b->contStack[q].p != st) dropGraph(b);
+	}
+	return rc;
+}
+
 WfParams params(mrad_batch *b) {
 	WfParams P{};
 	P.S = b->external ? b->externalView : b->arena.view;
@@ -1013,6 +1269,11 @@ WfParams params(mrad_batch *b) {
 	P.chunkPaths = b->chunkPaths;
 	P.capacity = (uint32_t) b->cfg.queue_capacity;
 	P.stats = b->cfg.collect_metrics ? b->stats.as<unsigned long long>() : nullptr;
+	for (int q = 0; q < 2; q++) {
+		P.cont[q].hdr = b->contHdr[q].as<uint4>();
+		P.cont[q].stack = b->contStack[q].as<unsigned long long>();
+	}
+	P.contCap = cappedPasses(b->cfg) ? (uint32_t) b->cfg.suspend_capacity : 0u;
 	return P;
 }
 
@@ -1033,7 +1294,17 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 	const TraceFn trace = b->cfg.collect_metrics
 	                          ? traceFor<true>(stackStorage(b->cfg), b->cfg.lds_stack_entries, regroup)
 	                          : traceFor<false>(stackStorage(b->cfg), b->cfg.lds_stack_entries, regroup);
-	hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	if (cappedPasses(b->cfg)) {
+		const TracePassFn pass = b->cfg.collect_metrics
+		                             ? tracePassFor<true>(stackStorage(b->cfg), b->cfg.lds_stack_entries)
+		                             : tracePassFor<false>(stackStorage(b->cfg), b->cfg.lds_stack_entries);
+		for (int p = 0; p < b->cfg.trace_passes; p++) {
+			const int budget = p + 1 < b->cfg.trace_passes ? b->cfg.pass_pops : 0;
+			hipLaunchKernelGGL(pass, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur, p, budget);
+		}
+	} else {
+		hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+	}
 	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
 	if (!fusedShade(b->cfg)) hipLaunchKernelGGL(wfShade, dim3(shadeBlocks), dim3(kBlock), 0, b->stream, P, cur);
 	hipLaunchKernelGGL(wfAdvanceShade, dim3(1), dim3(64), 0, b->stream, P, cur);
@@ -1075,6 +1346,15 @@ int normalizeConfig(mrad_wavefront_config &c) {
 	if (c.result_chunk_paths < 0 || c.result_chunk_paths > (1 << 30))
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "result chunk paths must be in [0, 2^30]");
 	c.queue_capacity = (c.queue_capacity + 63) & ~63;
+	if (c.trace_passes == 0) c.trace_passes = kDefaultTracePasses;
+	if (c.pass_pops == 0) c.pass_pops = kDefaultPassPops;
+	if (c.suspend_capacity == 0) c.suspend_capacity = std::max(64, c.queue_capacity / 2);
+	if (c.trace_passes < 1 || c.trace_passes > 3)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "trace passes must be 1, 2 or 3");
+	if (c.pass_pops < 1 || c.pass_pops > (1 << 20))
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "pass pops must be in [1, 2^20]");
+	if (c.suspend_capacity < 64 || c.suspend_capacity > c.queue_capacity)
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "suspend capacity must be in [64, queue capacity]");
 	return MRAD_OK;
 }
 
@@ -1105,6 +1385,7 @@ int renderLocked(mrad_batch *b) {
 	const int64_t t0 = nowNanos();
 	const bool wasDirty = b->dirty;
 	MRAD_TRY(prepare(b));
+	MRAD_TRY(ensureSuspend(b));
 	b->last.upload_nanos = wasDirty ? nowNanos() - t0 : 0;
 	const WfParams P = params(b);
 	const bool timed = b->cfg.time_kernels != 0;

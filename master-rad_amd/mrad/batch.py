@@ -29,6 +29,9 @@ class WavefrontConfig:
     leaf_regroup: bool = False
     result_chunk_paths: int = 0
     stack_storage: int = 0
+    trace_passes: int = 0        # closest-hit passes per bounce step (0 = the library default)
+    pass_pops: int = 0           # pops of a capped pass (0 = the library default)
+    suspend_capacity: int = 0    # suspended queries per pass (0 = half the queue)
     device: int = 0
 
     def native(self) -> N.WavefrontConfig:
@@ -36,7 +39,7 @@ class WavefrontConfig:
                                  1 if self.linear_scan else 0, 1 if self.collect_metrics else 0,
                                  1 if self.use_graph else 0, 1 if self.time_kernels else 0,
                                  self.lds_stack_entries, 1 if self.leaf_regroup else 0, self.result_chunk_paths,
-                                 self.stack_storage)
+                                 self.stack_storage, self.trace_passes, self.pass_pops, self.suspend_capacity)
 
 
 def make_view(camera, width: int, height: int, samples: int, max_depth: int, frame_index: int, frame_seed: int,

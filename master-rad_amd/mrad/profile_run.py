@@ -48,12 +48,18 @@ def _het(spp: int):
     b.close()
 
 
+# The wavefront configuration of the batch-64 profiles (kernel trace, SQ
+# counters, HBM traffic): bench.py times its roofline render with the same
+# one, so per-dispatch counters and times describe the same kernel launches.
+BATCH_PROFILE_CONFIG = dict(queue_capacity=1 << 26, waves_per_cu=28, trace_passes=2, pass_pops=128)
+
+
 def _batch():
     import mrad
     from mrad import configs
     ws = configs.batch(64, threads=16)
     print("batch built", flush=True)
-    b = mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 26))
+    b = mrad.BatchRenderer(mrad.WavefrontConfig(**BATCH_PROFILE_CONFIG))
     for w in ws:
         b.add_scene(w.scene, w.camera, w.width, w.height, w.spp, w.max_depth, 0, w.frame_seed, w.rr_start)
     st = b.render()

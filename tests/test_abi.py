@@ -99,6 +99,20 @@ def test_leaf_batch_setter_checks_arguments():
     assert b"null" in N.lib.mrad_last_error()
 
 
+def test_trace_pass_settings_are_checked_at_creation():
+    """trace_passes 1..3, pass_pops in [1, 2^20] and suspend_capacity in [64, queue_capacity] (0 picks the
+    default); checked before any device work, so no GPU is needed."""
+    for ok in (dict(trace_passes=2), dict(trace_passes=3, pass_pops=1, suspend_capacity=64),
+               dict(trace_passes=2, pass_pops=1 << 20, suspend_capacity=4096)):
+        mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=4096, **ok)).close()
+    for bad, msg in ((dict(trace_passes=4), "trace passes"), (dict(trace_passes=-1), "trace passes"),
+                     (dict(pass_pops=-5), "pass pops"), (dict(pass_pops=(1 << 20) + 1), "pass pops"),
+                     (dict(suspend_capacity=32), "suspend capacity"),
+                     (dict(suspend_capacity=4097), "suspend capacity")):
+        with pytest.raises(mrad.InvalidArgument, match=msg):
+            mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=4096, **bad))
+
+
 def test_destroy_null_is_noop():
     N.lib.mrad_destroy(None)
     N.lib.mrad_scene_destroy(None)

@@ -161,3 +161,31 @@ def test_batch_of_more_than_2_32_paths(require_gpu):
         for k in range(2):
             ref = _mega(sc, cam, w, h, spp, depth, k, seeds[k])[0]
             np.testing.assert_array_equal(b.image(k).view(np.uint32), ref.view(np.uint32))
+
+
+PASS_CASES = [(lambda: small_field(4000, 5, clustered=True), configs.field_camera, 80, 45, 3, 6, -1),
+              (gi_test_scene, gi_camera, 40, 40, 2, 12, -1),
+              (lambda: small_field(3000), configs.field_camera, 64, 36, 4, 8, 3)]
+
+
+@pytest.mark.parametrize("passes,pops,cap", [(2, 4, 0), (3, 2, 0), (3, 8, 64), (2, 1, 128), (2, 128, 0)])
+def test_capped_trace_passes_equal_single_pass(require_gpu, passes, pops, cap):
+    """Closest-hit queries suspended after `pops` pops and resumed by the next
+    pass give the single pass's images and all 14 counters; tiny budgets
+    suspend most queries several times, and a 64- or 128-item list overflows
+    (those queries finish in their own pass)."""
+    for k, (mk_scene, mk_cam, w, h, spp, depth, rr) in enumerate(PASS_CASES):
+        sc, cam = mk_scene(), mk_cam()
+        seed = mrad.frame_seed(91 + k, 0)
+        out = {}
+        for name, extra in (("single", dict(trace_passes=1)),
+                            ("passes", dict(trace_passes=passes, pass_pops=pops, suspend_capacity=cap))):
+            cfg = mrad.WavefrontConfig(queue_capacity=5056, collect_metrics=True, **extra)
+            with mrad.BatchRenderer(cfg) as b:
+                b.add_scene(sc, cam, w, h, spp, depth, 0, seed, rr)
+                st = b.render()
+                out[name] = (b.image(0).copy(), st)
+        print(f"case {k} passes={passes} pops={pops} cap={cap}: rays {out['single'][1]['rays']}", flush=True)
+        np.testing.assert_array_equal(out["passes"][0].view(np.uint32), out["single"][0].view(np.uint32))
+        for n in mrad.STAT_NAMES:
+            assert out["passes"][1][n] == out["single"][1][n], n
