@@ -336,6 +336,15 @@ __device__ __forceinline__ uint32_t fetchTraceChunk(uint32_t *next, uint32_t *qu
 	}
 }
 
+// Threads per block of the trace kernels (wfTrace, wfTracePass): the waves
+// of a block share its super-chunks (and its CU's L1).
+#ifndef MRAD_TRACE_BLOCK
+#define MRAD_TRACE_BLOCK 256
+#endif
+constexpr int kTraceBlock = MRAD_TRACE_BLOCK;
+constexpr int kTraceWaves = kTraceBlock / 64;
+static_assert(kTraceBlock % 64 == 0 && kTraceBlock <= 1024, "trace blocks hold whole waves");
+
 // Occupancy the trace kernels are compiled for (waves per SIMD; the register
 // budget is 512 / this).  7 waves (72 VGPRs) with the packed stack slots and
 // the branch-free push is 1.8 % faster than 8 waves (64 VGPRs), where both
@@ -365,12 +374,12 @@ __device__ __forceinline__ int octantOf(float dx, float dy, float dz) {
 // their radiance and are counted per result buffer as wfShade counts them.
 // The hit records are then never written.
 template <bool kMetrics, int kCap, int kShort, bool kFuse = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
 void wfTrace(WfParams P, int cur) {
-	__shared__ StackLdsT<kShort> stackLds;
+	__shared__ StackLdsT<kShort, kTraceWaves> stackLds;
 	const LdsStack lds = laneStack(stackLds);
 	constexpr int kTop = kTraceTop;
-	__shared__ float4 topLds[kWaves][kTop > 0 ? 4 * kTop : 1];
+	__shared__ float4 topLds[kTraceWaves][kTop > 0 ? 4 * kTop : 1];
 	__shared__ uint32_t doneLds[2];
 	__shared__ uint32_t nextLds;
 	const uint32_t count = P.counters[kCount0 + cur];
@@ -537,9 +546,9 @@ __device__ __forceinline__ void bvhLoopAnyForm(const DevScene &S, const SceneRan
 // Pass `pass` (0, 1 or 2) of a bounce step on pool `cur`; budget <= 0 for the
 // last pass.  Pass p > 0 reads list (p - 1) & 1; pass p writes list p & 1.
 template <bool kMetrics, int kCap, int kShort>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
 void wfTracePass(WfParams P, int cur, int pass, int budget) {
-	__shared__ StackLdsT<kShort> stackLds;
+	__shared__ StackLdsT<kShort, kTraceWaves> stackLds;
 	const LdsStack lds = laneStack(stackLds);
 	const int lane = threadIdx.x & 63;
 	const unsigned long long below = (1ull << lane) - 1ull;
@@ -1282,6 +1291,7 @@ WfParams params(mrad_batch *b) {
 // kernel boundaries are recorded for per-kernel timing.
 int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = nullptr) {
 	const int persistentBlocks = std::max(1, b->cuCount * b->cfg.waves_per_cu / kWaves);
+	const int traceBlocks = std::max(1, b->cuCount * b->cfg.waves_per_cu / kTraceWaves);
 	// shading takes fixed tiles: one resident wave of blocks, whatever the
 	// trace kernel's setting
 	const int shadeBlocks = std::max(1, b->cuCount * b->shadeBlocksPerCu);
@@ -1300,10 +1310,11 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 		                             : tracePassFor<false>(stackStorage(b->cfg), b->cfg.lds_stack_entries);
 		for (int p = 0; p < b->cfg.trace_passes; p++) {
 			const int budget = p + 1 < b->cfg.trace_passes ? b->cfg.pass_pops : 0;
-			hipLaunchKernelGGL(pass, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur, p, budget);
+			hipLaunchKernelGGL(pass, dim3(traceBlocks), dim3(kTraceBlock), 0, b->stream, P, cur, p, budget);
 		}
 	} else {
-		hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+		if (regroup) hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
+		else hipLaunchKernelGGL(trace, dim3(traceBlocks), dim3(kTraceBlock), 0, b->stream, P, cur);
 	}
 	if (ev) MRAD_HIP(hipEventRecord(ev[2], b->stream));
 	if (!fusedShade(b->cfg)) hipLaunchKernelGGL(wfShade, dim3(shadeBlocks), dim3(kBlock), 0, b->stream, P, cur);
