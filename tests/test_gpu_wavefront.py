@@ -189,3 +189,23 @@ def test_capped_trace_passes_equal_single_pass(require_gpu, passes, pops, cap):
         np.testing.assert_array_equal(out["passes"][0].view(np.uint32), out["single"][0].view(np.uint32))
         for n in mrad.STAT_NAMES:
             assert out["passes"][1][n] == out["single"][1][n], n
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+def test_static_super_chunks_equal_megakernel(require_gpu, passes):
+    """A grid of one wave per CU (64 blocks) and a 256x256, 8-spp frame (524,288 paths in a 2^20 pool) give the
+    trace kernel whole static rounds of 2048-path super-chunks (70 % of the pool, interleaved over the blocks)
+    and a global tail; images and all 14 counters equal the megakernel's, in one pass and in two."""
+    sc, cam = small_field(3000), configs.field_camera()
+    seed = mrad.frame_seed(0x5C, 0)
+    ref, ref_stats = _mega(sc, cam, 256, 256, 8, 6, 0, seed, rr=3, metrics=True)
+    for metrics in (False, True):
+        cfg = mrad.WavefrontConfig(queue_capacity=1 << 20, waves_per_cu=1, collect_metrics=metrics,
+                                   trace_passes=passes, pass_pops=24)
+        with mrad.BatchRenderer(cfg) as b:
+            b.add_scene(sc, cam, 256, 256, 8, 6, 0, seed, 3)
+            st = b.render()
+            np.testing.assert_array_equal(b.image(0).view(np.uint32), ref.view(np.uint32))
+        if metrics:
+            for k in mrad.STAT_NAMES:
+                assert st[k] == ref_stats[k], k
