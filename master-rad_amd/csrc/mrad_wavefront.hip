@@ -289,7 +289,7 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 // consecutive paths (neighbouring pixels, or paths that were neighbours a
 // bounce earlier), its waves taking 64-path chunks of the current one in
 // turn from a block counter in LDS, so that their node fetches hit the same
-// lines.  The first kTraceStaticPct % of the pool (whole rounds of
+// lines.  The first MRAD_TRACE_STATIC_PCT % of the pool (whole rounds of
 // super-chunks) is dealt out statically and interleaved: block b takes
 // super-chunks b, b + B, b + 2B, ... of the B blocks, so every block's share
 // samples the whole pool and the shares cost about the same.  The rest is a
@@ -299,7 +299,7 @@ __device__ __forceinline__ uint32_t fetchChunk(uint32_t *counter) {
 #define MRAD_TRACE_SUPER 2048
 #endif
 #ifndef MRAD_TRACE_STATIC_PCT
-#define MRAD_TRACE_STATIC_PCT 70
+#define MRAD_TRACE_STATIC_PCT 60
 #endif
 constexpr uint32_t kTraceSuper = MRAD_TRACE_SUPER;
 static_assert(kTraceSuper % 64 == 0, "super-chunks hold whole 64-path chunks");

@@ -194,7 +194,7 @@ def test_capped_trace_passes_equal_single_pass(require_gpu, passes, pops, cap):
 @pytest.mark.parametrize("passes", [1, 2])
 def test_static_super_chunks_equal_megakernel(require_gpu, passes):
     """A grid of one wave per CU (64 blocks) and a 256x256, 8-spp frame (524,288 paths in a 2^20 pool) give the
-    trace kernel whole static rounds of 2048-path super-chunks (70 % of the pool, interleaved over the blocks)
+    trace kernel whole static rounds of 2048-path super-chunks (60 % of the pool, interleaved over the blocks)
     and a global tail; images and all 14 counters equal the megakernel's, in one pass and in two."""
     sc, cam = small_field(3000), configs.field_camera()
     seed = mrad.frame_seed(0x5C, 0)
