@@ -242,7 +242,7 @@ def bench_batch(args) -> dict:
         t1 = time.perf_counter()
         grid = {"queue_capacity": [1 << 24, 1 << 25, 1 << 26], "waves_per_cu": [24, 28, 32],
                 "lds_stack_entries": [4, 10], "stack_storage": [32, 64], "trace_passes": [1, 2],
-                "pass_pops": [96, 128, 192]}
+                "pass_pops": [64, 96, 128]}
         best, table = tune(b, grid=grid, log=log, budget_s=args.autotune_budget)
         cfg = dict(queue_capacity=best.queue_capacity, waves_per_cu=best.waves_per_cu,
                    lds_stack_entries=best.lds_stack_entries, stack_storage=best.stack_storage,
@@ -670,7 +670,7 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=28)
     ap.add_argument("--trace-passes", type=int, default=2,
                     help="closest-hit passes per bounce step of the batch (capped passes regroup long traversals)")
-    ap.add_argument("--pass-pops", type=int, default=128, help="pop budget of a capped trace pass")
+    ap.add_argument("--pass-pops", type=int, default=96, help="pop budget of a capped trace pass")
     ap.add_argument("--random-layouts", type=int, default=10,
                     help="layouts per random F1-F4 row timed by --workload hero (the study used 100)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

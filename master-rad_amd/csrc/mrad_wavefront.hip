@@ -525,6 +525,9 @@ void wfTrace(WfParams P, int cur) {
 // hits, images and counters are those of the single pass.  A query whose
 // stack holds more than kSuspendEntries entries, or that finds its list full,
 // is not suspended: it keeps traversing in the current pass.
+#ifndef MRAD_PASS_CAMERA_UNCAPPED
+#define MRAD_PASS_CAMERA_UNCAPPED 1
+#endif
 #ifndef MRAD_SUSPEND_ENTRIES
 #define MRAD_SUSPEND_ENTRIES 12
 #endif
@@ -618,6 +621,12 @@ void wfTracePass(WfParams P, int cur, int pass, int budget) {
 		}
 		bool suspended = false;
 		int left = budget > 0 ? budget : 0x7FFFFFFF;
+#if MRAD_PASS_CAMERA_UNCAPPED
+		// a wave of camera paths only is not capped: camera rays of a wave
+		// (neighbouring pixels, or one pixel's samples) traverse alike, and
+		// suspending them costs their locality for little regrouping
+		if (pass == 0 && __ballot(live && (meta.z >> 16) != 0u) == 0ull) left = 0x7FFFFFFF;
+#endif
 		for (;;) {
 			if (traversing) bvhLoopAnyForm<kMetrics, kCap, kShort>(P.S, R, ray, best, c, b, left);
 			const bool open = traversing && !b.overflow && b.size > 0;
@@ -1118,10 +1127,10 @@ TraceFn traceFor(int storage, int ldsEntries, bool regroup) {
 
 // The capped passes (trace_passes > 1) of the plain, unfused trace.
 #ifndef MRAD_TRACE_PASSES
-#define MRAD_TRACE_PASSES 1
+#define MRAD_TRACE_PASSES 2
 #endif
 #ifndef MRAD_PASS_POPS
-#define MRAD_PASS_POPS 128
+#define MRAD_PASS_POPS 96
 #endif
 constexpr int kDefaultTracePasses = MRAD_TRACE_PASSES;
 constexpr int kDefaultPassPops = MRAD_PASS_POPS;

@@ -23,7 +23,7 @@ GRID = {
     "lds_stack_entries": [4, 10],
     "stack_storage": [32, 64],
     "trace_passes": [1, 2],
-    "pass_pops": [96, 128, 192],
+    "pass_pops": [64, 96, 128],
 }
 
 
