@@ -234,7 +234,7 @@ def bench_batch(args) -> dict:
     prims = sum(w.scene.primitive_count() for w in ws)
     log(f"built 64 scenes ({prims} primitives, {total_paths} paths) in {time.perf_counter() - t0:.1f} s")
     cfg = dict(queue_capacity=args.queue_capacity, waves_per_cu=args.waves_per_cu, trace_passes=args.trace_passes,
-               pass_pops=args.pass_pops)
+               pass_pops=args.pass_pops, pass_dense_lanes=args.pass_dense_lanes)
     b = make_batch(ws, **cfg)
     tuned = None
     if args.autotune_budget > 0:
@@ -242,11 +242,11 @@ def bench_batch(args) -> dict:
         t1 = time.perf_counter()
         grid = {"queue_capacity": [1 << 24, 1 << 25, 1 << 26], "waves_per_cu": [24, 28, 32],
                 "lds_stack_entries": [4, 10], "stack_storage": [32, 64], "trace_passes": [1, 2],
-                "pass_pops": [64, 96, 128]}
+                "pass_pops": [64, 80, 96], "pass_dense_lanes": [-1, 16, 32]}
         best, table = tune(b, grid=grid, log=log, budget_s=args.autotune_budget)
         cfg = dict(queue_capacity=best.queue_capacity, waves_per_cu=best.waves_per_cu,
                    lds_stack_entries=best.lds_stack_entries, stack_storage=best.stack_storage,
-                   trace_passes=best.trace_passes, pass_pops=best.pass_pops)
+                   trace_passes=best.trace_passes, pass_pops=best.pass_pops, pass_dense_lanes=best.pass_dense_lanes)
         tuned = {"best": cfg, "grid": {k: v for k, v in grid.items()}, "trials": table,
                  "budget_seconds": args.autotune_budget, "seconds": time.perf_counter() - t1}
         log(f"autotuned in {tuned['seconds']:.0f} s: {cfg}")
@@ -670,7 +670,9 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=28)
     ap.add_argument("--trace-passes", type=int, default=2,
                     help="closest-hit passes per bounce step of the batch (capped passes regroup long traversals)")
-    ap.add_argument("--pass-pops", type=int, default=96, help="pop budget of a capped trace pass")
+    ap.add_argument("--pass-pops", type=int, default=80, help="pop budget of a capped trace pass")
+    ap.add_argument("--pass-dense-lanes", type=int, default=16,
+                    help="lanes still traversing at the pop budget that keep a wave going (-1: always suspend)")
     ap.add_argument("--random-layouts", type=int, default=10,
                     help="layouts per random F1-F4 row timed by --workload hero (the study used 100)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -288,9 +288,12 @@ typedef struct {
 	                            suspends the queries still traversing after pass_pops pops (waves of camera paths
 	                            only are not capped), and the next pass resumes them packed into full waves; images
 	                            and counters are the same */
-	int pass_pops;           /* traversal-stack pops of a capped pass, in [1, 2^20] (0 = 96) */
+	int pass_pops;           /* traversal-stack pops of a capped pass, in [1, 2^20] (0 = 80) */
 	int suspend_capacity;    /* suspended queries one pass can hold, in [64, queue_capacity] (0 = half the
 	                            queue); a query that finds the list full finishes in its pass */
+	int pass_dense_lanes;    /* a wave with at least this many lanes still traversing when its pop budget runs
+	                            out is not suspended: it keeps going and is checked again every 32 pops; in
+	                            [1, 64], -1 = every wave is suspended at the budget (0 = 16) */
 } mrad_wavefront_config;
 
 typedef struct {

@@ -528,6 +528,15 @@ void wfTrace(WfParams P, int cur) {
 #ifndef MRAD_PASS_CAMERA_UNCAPPED
 #define MRAD_PASS_CAMERA_UNCAPPED 1
 #endif
+// A wave that still has `dense` or more lanes traversing when its budget runs
+// out is not split: it keeps going for MRAD_PASS_RECHECK_POPS more pops and is
+// checked again, so only waves that have thinned out hand their queries to the
+// next pass (dense <= 0: every wave is suspended at the budget).
+#ifndef MRAD_PASS_RECHECK_POPS
+#define MRAD_PASS_RECHECK_POPS 32
+#endif
+constexpr int kPassRecheckPops = MRAD_PASS_RECHECK_POPS;
+static_assert(kPassRecheckPops >= 1, "dense-wave recheck interval");
 #ifndef MRAD_SUSPEND_ENTRIES
 #define MRAD_SUSPEND_ENTRIES 12
 #endif
@@ -548,9 +557,10 @@ __device__ __forceinline__ void bvhLoopAnyForm(const DevScene &S, const SceneRan
 
 // Pass `pass` (0, 1 or 2) of a bounce step on pool `cur`; budget <= 0 for the
 // last pass.  Pass p > 0 reads list (p - 1) & 1; pass p writes list p & 1.
+// `dense`: the dense-wave threshold above.
 template <bool kMetrics, int kCap, int kShort>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(MRAD_TRACE_WAVES_PER_EU)))
-void wfTracePass(WfParams P, int cur, int pass, int budget) {
+void wfTracePass(WfParams P, int cur, int pass, int budget, int dense) {
 	__shared__ StackLdsT<kShort, kTraceWaves> stackLds;
 	const LdsStack lds = laneStack(stackLds);
 	const int lane = threadIdx.x & 63;
@@ -630,6 +640,10 @@ void wfTracePass(WfParams P, int cur, int pass, int budget) {
 		for (;;) {
 			if (traversing) bvhLoopAnyForm<kMetrics, kCap, kShort>(P.S, R, ray, best, c, b, left);
 			const bool open = traversing && !b.overflow && b.size > 0;
+			if (dense > 0 && left != 0x7FFFFFFF && __popcll(__ballot(open)) >= dense) {
+				left = kPassRecheckPops;   // still dense: every open lane pops again before the next check
+				continue;
+			}
 			const bool want = open && b.size <= kSuspendEntries;
 			const unsigned long long m = __ballot(want);
 			uint32_t first = 0;
@@ -1130,11 +1144,15 @@ TraceFn traceFor(int storage, int ldsEntries, bool regroup) {
 #define MRAD_TRACE_PASSES 2
 #endif
 #ifndef MRAD_PASS_POPS
-#define MRAD_PASS_POPS 96
+#define MRAD_PASS_POPS 80
 #endif
 constexpr int kDefaultTracePasses = MRAD_TRACE_PASSES;
 constexpr int kDefaultPassPops = MRAD_PASS_POPS;
-using TracePassFn = void (*)(WfParams, int, int, int);
+#ifndef MRAD_PASS_DENSE_LANES
+#define MRAD_PASS_DENSE_LANES 16
+#endif
+constexpr int kDefaultPassDenseLanes = MRAD_PASS_DENSE_LANES;
+using TracePassFn = void (*)(WfParams, int, int, int, int);
 inline bool cappedPasses(const mrad_wavefront_config &c) {
 	return c.trace_passes > 1 && c.leaf_regroup == 0 && !fusedShade(c);
 }
@@ -1319,7 +1337,8 @@ int enqueueStep(mrad_batch *b, const WfParams &P, int cur, hipEvent_t *ev = null
 		                             : tracePassFor<false>(stackStorage(b->cfg), b->cfg.lds_stack_entries);
 		for (int p = 0; p < b->cfg.trace_passes; p++) {
 			const int budget = p + 1 < b->cfg.trace_passes ? b->cfg.pass_pops : 0;
-			hipLaunchKernelGGL(pass, dim3(traceBlocks), dim3(kTraceBlock), 0, b->stream, P, cur, p, budget);
+			hipLaunchKernelGGL(pass, dim3(traceBlocks), dim3(kTraceBlock), 0, b->stream, P, cur, p, budget,
+			                   b->cfg.pass_dense_lanes);
 		}
 	} else {
 		if (regroup) hipLaunchKernelGGL(trace, dim3(persistentBlocks), dim3(kBlock), 0, b->stream, P, cur);
@@ -1369,12 +1388,15 @@ int normalizeConfig(mrad_wavefront_config &c) {
 	if (c.trace_passes == 0) c.trace_passes = kDefaultTracePasses;
 	if (c.pass_pops == 0) c.pass_pops = kDefaultPassPops;
 	if (c.suspend_capacity == 0) c.suspend_capacity = std::max(64, c.queue_capacity / 2);
+	if (c.pass_dense_lanes == 0) c.pass_dense_lanes = kDefaultPassDenseLanes;
 	if (c.trace_passes < 1 || c.trace_passes > 3)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "trace passes must be 1, 2 or 3");
 	if (c.pass_pops < 1 || c.pass_pops > (1 << 20))
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "pass pops must be in [1, 2^20]");
 	if (c.suspend_capacity < 64 || c.suspend_capacity > c.queue_capacity)
 		return set_error(MRAD_ERR_INVALID_ARGUMENT, "suspend capacity must be in [64, queue capacity]");
+	if (c.pass_dense_lanes != -1 && (c.pass_dense_lanes < 1 || c.pass_dense_lanes > 64))
+		return set_error(MRAD_ERR_INVALID_ARGUMENT, "pass dense lanes must be -1 or in [1, 64]");
 	return MRAD_OK;
 }
 

@@ -119,7 +119,8 @@ class WavefrontConfig(C.Structure):
                 ("bvh_stack_size", C.c_int), ("linear_scan", C.c_int), ("collect_metrics", C.c_int),
                 ("use_graph", C.c_int), ("time_kernels", C.c_int), ("lds_stack_entries", C.c_int),
                 ("leaf_regroup", C.c_int), ("result_chunk_paths", C.c_int), ("stack_storage", C.c_int),
-                ("trace_passes", C.c_int), ("pass_pops", C.c_int), ("suspend_capacity", C.c_int)]
+                ("trace_passes", C.c_int), ("pass_pops", C.c_int), ("suspend_capacity", C.c_int),
+                ("pass_dense_lanes", C.c_int)]
 
 
 class BatchStats(C.Structure):

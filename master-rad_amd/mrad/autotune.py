@@ -3,8 +3,9 @@
 Coordinate descent over queue capacity (paths in flight), waves per CU of
 the persistent kernels, the LDS split of the traversal stack (entries per
 lane in LDS), the stack depth (entries per lane of the ahead-of-time stack
-storage) and the closest-hit passes per bounce step with their pop budget
-(capped passes regroup long traversals into full waves), on a loaded batch: each candidate is applied with
+storage), the closest-hit passes per bounce step with their pop budget
+(capped passes regroup long traversals into full waves) and the lanes still
+traversing that keep a wave from being suspended, on a loaded batch: each candidate is applied with
 mrad_batch_configure (scenes stay resident), rendered once to warm and then
 timed.  Images do not depend on these parameters (every path runs the same
 arithmetic), so only time is compared.  A time budget bounds the search: once
@@ -23,7 +24,8 @@ GRID = {
     "lds_stack_entries": [4, 10],
     "stack_storage": [32, 64],
     "trace_passes": [1, 2],
-    "pass_pops": [64, 96, 128],
+    "pass_pops": [64, 80, 96],
+    "pass_dense_lanes": [-1, 16, 32],
 }
 
 

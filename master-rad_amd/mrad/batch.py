@@ -32,6 +32,8 @@ class WavefrontConfig:
     trace_passes: int = 0        # closest-hit passes per bounce step (0 = the library default)
     pass_pops: int = 0           # pops of a capped pass (0 = the library default)
     suspend_capacity: int = 0    # suspended queries per pass (0 = half the queue)
+    pass_dense_lanes: int = 0    # lanes still traversing that keep a wave going at the budget (0 = the library
+                                 # default, -1 = every wave suspends)
     device: int = 0
 
     def native(self) -> N.WavefrontConfig:
@@ -39,7 +41,8 @@ class WavefrontConfig:
                                  1 if self.linear_scan else 0, 1 if self.collect_metrics else 0,
                                  1 if self.use_graph else 0, 1 if self.time_kernels else 0,
                                  self.lds_stack_entries, 1 if self.leaf_regroup else 0, self.result_chunk_paths,
-                                 self.stack_storage, self.trace_passes, self.pass_pops, self.suspend_capacity)
+                                 self.stack_storage, self.trace_passes, self.pass_pops, self.suspend_capacity,
+                                 self.pass_dense_lanes)
 
 
 def make_view(camera, width: int, height: int, samples: int, max_depth: int, frame_index: int, frame_seed: int,

@@ -51,7 +51,8 @@ def _het(spp: int):
 # The wavefront configuration of the batch-64 profiles (kernel trace, SQ
 # counters, HBM traffic): bench.py times its roofline render with the same
 # one, so per-dispatch counters and times describe the same kernel launches.
-BATCH_PROFILE_CONFIG = dict(queue_capacity=1 << 26, waves_per_cu=28, trace_passes=2, pass_pops=96)
+BATCH_PROFILE_CONFIG = dict(queue_capacity=1 << 26, waves_per_cu=28, trace_passes=2, pass_pops=80,
+                            pass_dense_lanes=16)
 
 
 def _batch():

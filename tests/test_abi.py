@@ -100,15 +100,18 @@ def test_leaf_batch_setter_checks_arguments():
 
 
 def test_trace_pass_settings_are_checked_at_creation():
-    """trace_passes 1..3, pass_pops in [1, 2^20] and suspend_capacity in [64, queue_capacity] (0 picks the
-    default); checked before any device work, so no GPU is needed."""
-    for ok in (dict(trace_passes=2), dict(trace_passes=3, pass_pops=1, suspend_capacity=64),
-               dict(trace_passes=2, pass_pops=1 << 20, suspend_capacity=4096)):
+    """trace_passes 1..3, pass_pops in [1, 2^20], suspend_capacity in [64, queue_capacity] and
+    pass_dense_lanes -1 or in [1, 64] (0 picks the default); checked before any device work, so no GPU is
+    needed."""
+    for ok in (dict(trace_passes=2), dict(trace_passes=3, pass_pops=1, suspend_capacity=64, pass_dense_lanes=-1),
+               dict(trace_passes=2, pass_pops=1 << 20, suspend_capacity=4096, pass_dense_lanes=64),
+               dict(pass_dense_lanes=1)):
         mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=4096, **ok)).close()
     for bad, msg in ((dict(trace_passes=4), "trace passes"), (dict(trace_passes=-1), "trace passes"),
                      (dict(pass_pops=-5), "pass pops"), (dict(pass_pops=(1 << 20) + 1), "pass pops"),
                      (dict(suspend_capacity=32), "suspend capacity"),
-                     (dict(suspend_capacity=4097), "suspend capacity")):
+                     (dict(suspend_capacity=4097), "suspend capacity"),
+                     (dict(pass_dense_lanes=65), "pass dense lanes"), (dict(pass_dense_lanes=-2), "pass dense lanes")):
         with pytest.raises(mrad.InvalidArgument, match=msg):
             mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=4096, **bad))
 

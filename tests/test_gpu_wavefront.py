@@ -168,24 +168,29 @@ PASS_CASES = [(lambda: small_field(4000, 5, clustered=True), configs.field_camer
               (lambda: small_field(3000), configs.field_camera, 64, 36, 4, 8, 3)]
 
 
-@pytest.mark.parametrize("passes,pops,cap", [(2, 4, 0), (3, 2, 0), (3, 8, 64), (2, 1, 128), (2, 128, 0)])
-def test_capped_trace_passes_equal_single_pass(require_gpu, passes, pops, cap):
+@pytest.mark.parametrize("passes,pops,cap,dense", [(2, 4, 0, -1), (3, 2, 0, -1), (3, 8, 64, -1), (2, 1, 128, -1),
+                                                  (2, 128, 0, -1), (2, 4, 0, 0), (3, 2, 64, 8), (2, 16, 0, 64)])
+def test_capped_trace_passes_equal_single_pass(require_gpu, passes, pops, cap, dense):
     """Closest-hit queries suspended after `pops` pops and resumed by the next
     pass give the single pass's images and all 14 counters; tiny budgets
     suspend most queries several times, and a 64- or 128-item list overflows
-    (those queries finish in their own pass)."""
+    (those queries finish in their own pass).  With `dense` -1 every wave is
+    suspended at its budget; otherwise a wave with that many lanes still
+    traversing keeps going and is checked again (0: the library's threshold)."""
     for k, (mk_scene, mk_cam, w, h, spp, depth, rr) in enumerate(PASS_CASES):
         sc, cam = mk_scene(), mk_cam()
         seed = mrad.frame_seed(91 + k, 0)
         out = {}
         for name, extra in (("single", dict(trace_passes=1)),
-                            ("passes", dict(trace_passes=passes, pass_pops=pops, suspend_capacity=cap))):
+                            ("passes", dict(trace_passes=passes, pass_pops=pops, suspend_capacity=cap,
+                                            pass_dense_lanes=dense))):
             cfg = mrad.WavefrontConfig(queue_capacity=5056, collect_metrics=True, **extra)
             with mrad.BatchRenderer(cfg) as b:
                 b.add_scene(sc, cam, w, h, spp, depth, 0, seed, rr)
                 st = b.render()
                 out[name] = (b.image(0).copy(), st)
-        print(f"case {k} passes={passes} pops={pops} cap={cap}: rays {out['single'][1]['rays']}", flush=True)
+        print(f"case {k} passes={passes} pops={pops} cap={cap} dense={dense}: rays {out['single'][1]['rays']}",
+              flush=True)
         np.testing.assert_array_equal(out["passes"][0].view(np.uint32), out["single"][0].view(np.uint32))
         for n in mrad.STAT_NAMES:
             assert out["passes"][1][n] == out["single"][1][n], n
