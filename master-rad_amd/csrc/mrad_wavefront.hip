@@ -61,7 +61,9 @@ struct WfChunk {
 // The result slot is buffer * chunkPaths + the path's index in its chunk.
 // Trace reads ori, dir and meta; shade reads dir and meta, and tl only past
 // depth 0 (a camera path starts with T = 1 and L = 0, which generate does not
-// store); nothing but trace reads the origin.
+// store); nothing but trace reads the origin.  Generate stores no origin
+// either: every camera ray starts at its scene's camera position
+// (poolOrigin).
 struct WfPool {
 	float4 *ori, *dir, *tl;
 	uint4 *meta;
@@ -176,6 +178,16 @@ __device__ __forceinline__ void rankToPixel(uint32_t r, int width, int height, i
 	y = (int) (8u * band + ly);
 }
 
+// The origin of pool entry i: a camera path's (depth 0) is its scene's camera
+// position, cameraRay's origin, which generate does not store.
+__device__ __forceinline__ float4 poolOrigin(const WfParams &P, const WfPool &Q, uint32_t i, const uint4 &meta) {
+	if ((meta.z >> 16) == 0u) {
+		const float *c = P.scenes[meta.z & 0xFFFFu].cam;
+		return make_float4(c[0], c[1], c[2], 0.0f);
+	}
+	return Q.ori[i];
+}
+
 // Paths the next generate step appends to pool `cur`: up to the pool's free
 // space, from the current chunk only, and none from a chunk the host has not
 // released yet.
@@ -220,8 +232,7 @@ __global__ __launch_bounds__(kBlock) void wfGenerate(WfParams P, int cur) {
 			P.results[3 * (size_t) result + 2] = 0.0f;
 		}
 		const uint32_t slot = base + j;
-		Q.ori[slot] = make_float4(o.x, o.y, o.z, 0.0f);
-		Q.dir[slot] = make_float4(d.x, d.y, d.z, 1.0f);
+		Q.dir[slot] = make_float4(d.x, d.y, d.z, 1.0f);          // o is the camera position (poolOrigin)
 		Q.meta[slot] = make_uint4(rng, result, (uint32_t) k | (depth << 16), __float_as_uint(0.0f));
 	}
 }
@@ -423,7 +434,7 @@ void wfTrace(WfParams P, int cur) {
 					P.hitB[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 					continue;
 				}
-				const float4 a = Q.ori[i], b = Q.dir[i];
+				const float4 a = poolOrigin(P, Q, i, meta), b = Q.dir[i];
 				const f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
 				const SceneRange &R = P.scenes[meta.z & 0xFFFFu].R;
 				c.add(0, 1);
@@ -440,7 +451,7 @@ void wfTrace(WfParams P, int cur) {
 				if (depth == kDead) {
 					finished[buffer]++;
 				} else {
-					const float4 a = Q.ori[i], b = Q.dir[i];
+					const float4 a = poolOrigin(P, Q, i, meta), b = Q.dir[i];
 					f3 o = mk(a.x, a.y, a.z), d = mk(b.x, b.y, b.z);
 					const WfScene &V = P.scenes[meta.z & 0xFFFFu];
 					c.add(0, 1);
@@ -609,7 +620,7 @@ void wfTracePass(WfParams P, int cur, int pass, int budget, int dense) {
 		Ray ray{};
 		bool traversing = false;
 		if (live) {
-			const float4 a4 = Q.ori[i], d4 = Q.dir[i];
+			const float4 a4 = poolOrigin(P, Q, i, meta), d4 = Q.dir[i];
 			ray = makeRay(mk(a4.x, a4.y, a4.z), mk(d4.x, d4.y, d4.z), R.fastSlabs != 0);
 			if (pass == 0) {
 				c.add(0, 1);
@@ -934,7 +945,7 @@ __global__ __launch_bounds__(kBlock) void wfTraceRegroup(WfParams P, int cur) {
 		int size = 0;
 		bool overflow = false, trav = false;
 		if (valid) {
-			const float4 a = Q.ori[i], b = Q.dir[i];
+			const float4 a = poolOrigin(P, Q, i, meta), b = Q.dir[i];
 			ray = makeRay(mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), Rp->fastSlabs != 0);
 			c.add(0, 1);
 			const SceneRange &R = *Rp;
