@@ -240,7 +240,7 @@ def bench_batch(args) -> dict:
     if args.autotune_budget > 0:
         from mrad.autotune import tune
         t1 = time.perf_counter()
-        grid = {"queue_capacity": [1 << 24, 1 << 25, 1 << 26], "waves_per_cu": [24, 28, 32],
+        grid = {"queue_capacity": [1 << 25, 1 << 26, 1 << 27], "waves_per_cu": [24, 28, 32],
                 "lds_stack_entries": [4, 10], "stack_storage": [32, 64], "trace_passes": [1, 2],
                 "pass_pops": [64, 80, 96], "pass_dense_lanes": [-1, 16, 32]}
         best, table = tune(b, grid=grid, log=log, budget_s=args.autotune_budget)

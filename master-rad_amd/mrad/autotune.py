@@ -19,7 +19,7 @@ import time
 from .batch import BatchRenderer, WavefrontConfig
 
 GRID = {
-    "queue_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26],
+    "queue_capacity": [1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27],
     "waves_per_cu": [16, 24, 28, 32],
     "lds_stack_entries": [4, 10],
     "stack_storage": [32, 64],
