@@ -284,10 +284,10 @@ typedef struct {
 	                            result buffer (two are kept): 0 = 8 pools' worth in [2^24, 2^30]; at most 2^30 */
 	int stack_storage;       /* traversal-stack entries per lane compiled ahead of time: 32, 64 or 256, at least
 	                            bvh_stack_size (0 = 64, or 256 above a stack size of 64); the autotuned depth */
-	int trace_passes;        /* closest-hit passes per bounce step: 1, 2 or 3 (0 = 2).  Every pass but the last
-	                            suspends the queries still traversing after pass_pops pops (waves of camera paths
-	                            only are not capped), and the next pass resumes them packed into full waves; images
-	                            and counters are the same */
+	int trace_passes;        /* closest-hit passes per bounce step: 1, 2 or 3 (0 = 2 for a queue_capacity of 2^25
+	                            paths or more, else 1).  Every pass but the last suspends the queries still
+	                            traversing after pass_pops pops (waves of camera paths only are not capped), and the
+	                            next pass resumes them packed into full waves; images and counters are the same */
 	int pass_pops;           /* traversal-stack pops of a capped pass, in [1, 2^20] (0 = 80) */
 	int suspend_capacity;    /* suspended queries one pass can hold, in [64, queue_capacity] (0 = half the
 	                            queue); a query that finds the list full finishes in its pass */
