@@ -321,6 +321,10 @@ int mrad_batch_configure(mrad_batch *batch, const mrad_wavefront_config *config)
 /* Normalized RGB image (width*height*3 floats) of one scene. */
 int mrad_batch_copy_image(mrad_batch *batch, int scene, float *rgb_out);
 int mrad_batch_get_stats(mrad_batch *batch, mrad_batch_stats *out);
+/* The batch's launch parameters with every default resolved (the zero fields
+ * of the create or configure call replaced by the values in use, the queue
+ * capacity rounded up to whole waves).  No counterpart in the reference. */
+int mrad_batch_get_config(mrad_batch *batch, mrad_wavefront_config *out);
 
 /* ------------------------------------------- primitive-cost calibration */
 /* The measurement kernel of the reference's calibration campaign

@@ -1686,6 +1686,14 @@ int mrad_batch_get_stats(mrad_batch *b, mrad_batch_stats *out) {
 	return MRAD_OK;
 }
 
+int mrad_batch_get_config(mrad_batch *b, mrad_wavefront_config *out) {
+	clear_error();
+	if (!b || !out) return set_error(MRAD_ERR_INVALID_ARGUMENT, "batch and output are required");
+	std::lock_guard<std::mutex> g(b->mu);
+	*out = b->cfg;
+	return MRAD_OK;
+}
+
 }  // extern "C"
 
 namespace mrad {

@@ -137,6 +137,7 @@ SIGNATURES.update({
     "mrad_batch_configure": (C.c_int, [C.c_void_p, C.POINTER(WavefrontConfig)]),
     "mrad_batch_copy_image": (C.c_int, [C.c_void_p, C.c_int, fp]),
     "mrad_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "mrad_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(WavefrontConfig)]),
     "mrad_backward": (C.c_int, [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_uint64, fp, fp, C.c_int]),
     "mrad_update_materials": (C.c_int, [C.c_void_p, fp, C.c_int]),
     "mrad_cost_create": (C.c_int, [C.c_int, fp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p)]),

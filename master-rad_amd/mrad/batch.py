@@ -96,6 +96,12 @@ class BatchRenderer:
         N.check(N.lib.mrad_batch_configure(self._h, C.byref(config.native())))
         self.config = config
 
+    def effective_config(self) -> dict:
+        """The launch parameters in use, every default resolved (mrad_batch_get_config)."""
+        c = N.WavefrontConfig()
+        N.check(N.lib.mrad_batch_get_config(self._h, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in N.WavefrontConfig._fields_}
+
     def image(self, scene: int) -> np.ndarray:
         v = self.views[scene]
         out = np.zeros(v.width * v.height * 3, np.float32)

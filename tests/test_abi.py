@@ -116,6 +116,25 @@ def test_trace_pass_settings_are_checked_at_creation():
             mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=4096, **bad))
 
 
+def test_batch_config_defaults_are_resolved():
+    """mrad_batch_get_config reports the parameters in use: two capped trace passes of 80 pops (dense-wave
+    threshold 16, continuation lists of half the queue) for a queue of 2^25 paths or more, one pass below, the
+    queue rounded up to whole waves; explicit values are kept.  No device work, so no GPU is needed."""
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 22)) as b:
+        c = b.effective_config()
+        assert (c["trace_passes"], c["pass_pops"], c["pass_dense_lanes"]) == (1, 80, 16)
+        assert c["suspend_capacity"] == 1 << 21 and c["waves_per_cu"] == 28 and c["lds_stack_entries"] == 10
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 25)) as b:
+        c = b.effective_config()
+        assert (c["trace_passes"], c["pass_pops"], c["suspend_capacity"]) == (2, 80, 1 << 24)
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=(1 << 25) - 100, trace_passes=3, pass_pops=7,
+                                                 pass_dense_lanes=-1)) as b:
+        c = b.effective_config()
+        assert c["queue_capacity"] == (1 << 25) - 64
+        assert (c["trace_passes"], c["pass_pops"], c["pass_dense_lanes"]) == (3, 7, -1)
+    assert N.lib.mrad_batch_get_config(None, None) == N.MRAD_ERR_INVALID_ARGUMENT
+
+
 def test_destroy_null_is_noop():
     N.lib.mrad_destroy(None)
     N.lib.mrad_scene_destroy(None)

@@ -214,3 +214,20 @@ def test_static_super_chunks_equal_megakernel(require_gpu, passes):
         if metrics:
             for k in mrad.STAT_NAMES:
                 assert st[k] == ref_stats[k], k
+
+
+def test_configure_resolves_pass_defaults_for_the_new_queue(require_gpu):
+    """mrad_batch_configure resolves the zero fields for the new launch parameters: a 2^22-path batch traces in
+    one pass, and reconfigured to a 2^25-path queue in two; the image is the same either way."""
+    sc, cam = small_field(3000), configs.field_camera()
+    seed = mrad.frame_seed(0x77, 0)
+    with mrad.BatchRenderer(mrad.WavefrontConfig(queue_capacity=1 << 22)) as b:
+        b.add_scene(sc, cam, 96, 64, 4, 6, 0, seed, 3)
+        assert b.effective_config()["trace_passes"] == 1
+        b.render()
+        one = b.image(0).copy()
+        b.configure(mrad.WavefrontConfig(queue_capacity=1 << 25))
+        c = b.effective_config()
+        assert (c["trace_passes"], c["queue_capacity"], c["suspend_capacity"]) == (2, 1 << 25, 1 << 24)
+        b.render()
+        np.testing.assert_array_equal(b.image(0).view(np.uint32), one.view(np.uint32))
