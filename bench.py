@@ -220,6 +220,47 @@ def gi_test_context(frames: int = 20) -> dict:
             "published_cpu_segments": 4215129}
 
 
+# The reference's published platform comparison (benchmarks/results/platform-comparison.csv:2-7, RTX 3080):
+# scene, kernel mode, median kernel ms, GPU path segments of a frame.
+PUBLISHED_PLATFORM = [("GI_TEST", "uniform_bvh", 4.6029, 4215974), ("GI_TEST", "linear", 3.5724, 4215974),
+                      ("CITY_OF_NIGHT_V1", "uniform_bvh", 27.0259, 2552461),
+                      ("SIGNAL_CHAMBER", "uniform_bvh", 37.79205, 4461552)]
+
+
+def platform_comparison(frames: int = 10) -> list:
+    """The catalog scenes of the reference's CPU-vs-GPU platform comparison at its protocol (1280x720, 1 spp,
+    depth 12, uniform SAH leaf 8, SceneCatalog's cameras) in the megakernel, BVH or linear scan as the row says:
+    median kernel time and mean segments of a frame beside the published RTX 3080 figures.  The published
+    frame seeds are not recorded, so segments are a statistical cross-check."""
+    from mrad import workloads as W
+    make = {"GI_TEST": (W.gi_test_scene, W.gi_test_camera()),
+            "CITY_OF_NIGHT_V1": (W.city_of_night_scene, W.city_of_night_camera()),
+            "SIGNAL_CHAMBER": (W.signal_chamber_scene, W.signal_chamber_camera())}
+    wd, ht = W.GI_TEST_WIDTH, W.GI_TEST_HEIGHT
+    rows = []
+    for name, mode, pub_ms, pub_segs in PUBLISHED_PLATFORM:
+        scene_fn, camera = make[name]
+        sc = scene_fn(mrad.BvhBuildConfig.uniform(W.LEAF_SIZE))
+        cam = W.gpu_camera(camera, wd, ht)
+        bvh = mode != "linear"
+        kern, segs = [], []
+        with mrad.GpuRayTracer(wd, ht, 1, bvh) as t:
+            t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, 0, mrad.frame_seed(0x617E, 0))
+            for i in range(frames):
+                t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(0x617E, i))
+                kern.append(t.last_frame_stats()["kernel_nanos"] / 1e6)
+        with mrad.GpuRayTracer(wd, ht, 1, bvh, collect_metrics=True) as t:
+            for i in range(2):
+                t.render_sample(None, sc, cam, W.MAXIMUM_PATH_DEPTH, i, mrad.frame_seed(0x617E, i))
+                segs.append(t.last_frame_stats()["rays"])
+        k = float(np.median(kern))
+        rows.append({"scene": name, "mode": mode, "kernel_ms": k, "segments_per_frame": float(np.mean(segs)),
+                     "published_rtx3080_kernel_ms": pub_ms, "published_gpu_segments": pub_segs,
+                     "kernel_time_ratio": pub_ms / k})
+        log(f"platform comparison {name} {mode}: {k:.3f} ms (published RTX 3080 {pub_ms} ms)")
+    return rows
+
+
 def make_batch(ws, **cfg) -> mrad.BatchRenderer:
     b = mrad.BatchRenderer(mrad.WavefrontConfig(**cfg))
     for w in ws:
@@ -323,6 +364,7 @@ def bench_batch(args) -> dict:
     adj = bench_adjoint()
     gi = gi_test_context()
     log(f"GI_TEST context: {gi['kernel_ms']:.3f} ms kernel, {gi['segments_per_frame']:.0f} segments")
+    platform = platform_comparison()
     return {
         "metric": METRIC,
         "value": value,
@@ -355,6 +397,7 @@ def bench_batch(args) -> dict:
         "adjoint": adj,
         "adjoint_over_forward": adj["ratio"],
         "gi_test_context": gi,
+        "platform_comparison": platform,
         "roofline": {**roofline_entry(trace_kernel, trace_s / launches, launches, trace_s * 1e3,
                                       algorithmic_bytes(st_m) / launches, load_traffic("batch-64"),
                                       load_counters("batch-64"), "batch-64"), "render_config": prof_cfg},
